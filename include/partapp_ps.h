@@ -1,0 +1,177 @@
+/*
+ * partapp_ps — MI355X (gfx950) pictorial-structure inference, C ABI.
+ *
+ * Drop-in replacement for the rotation-aware sum-product sweep of partapp's
+ * libPictStruct (the `--find_obj` stage).  Every entry point takes plain
+ * pointers and sizes; device buffers belong to the caller, plans and their
+ * scratch belong to the library.  All functions return 0 on success or a
+ * negative ps_status; ps_last_error_string() describes the last failure of
+ * the calling thread.  Nothing aborts: the asserts of the reference become
+ * status codes.
+ *
+ * Layouts (C order, x fastest, as boost::multi_array in
+ * libMultiArray/multi_array_def.h:36 and objectdetect_findrot.cpp:494-495):
+ *   log unaries / log marginals of one instance: [P][S][R][H][W] float
+ *   root posterior of one instance:              [S][H][W]       float
+ *   MAP records:                                 [inst][S][P]    ps_map
+ */
+#ifndef PARTAPP_PS_H_
+#define PARTAPP_PS_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ps_status {
+  PS_OK = 0,
+  PS_ERR_INVALID_ARG = -1,   /* null pointer, bad size, bad flag            */
+  PS_ERR_TREE = -2,          /* not one root + chains (objectdetect_findrot.cpp:210,786;
+                                objectdetect_aux.cpp:131)                      */
+  PS_ERR_COVARIANCE = -3,    /* C diagonal not > 0 (multi_array_filter.hpp:219) */
+  PS_ERR_KERNEL_SIZE = -4,   /* Gaussian kernel >= 1000 taps (multi_array_filter.hpp:243-244) */
+  PS_ERR_UNSUPPORTED = -5,   /* e.g. off-diagonal C, R > 64                  */
+  PS_ERR_WARP = -6,          /* a nearest-neighbour warp is not a uniform shift over the grid */
+  PS_ERR_HIP = -7,           /* HIP runtime failure                          */
+  PS_ERR_NO_DEVICE = -8,     /* no usable gfx950 device                      */
+  PS_ERR_OUT_OF_MEMORY = -9
+} ps_status;
+
+/* ExpParam fields used on the path (libPartApp/ExpParam.proto:189-195,229);
+ * float / uint32 as in the proto because the reference's arithmetic depends on it. */
+typedef struct ps_exp_param {
+  float min_part_rotation;       /* degrees (default -180)          */
+  float max_part_rotation;       /* degrees (default  180)          */
+  unsigned num_rotation_steps;   /* R (default 48), 1..64           */
+  float min_object_scale;
+  float max_object_scale;
+  unsigned num_scale_steps;      /* S                               */
+  float strip_border_detections; /* fraction of width, 0 <= f < 0.5 */
+} ps_exp_param;
+
+/* object_detect::Joint after loadJoints (objectdetect.h:54-86,
+ * objectdetect_aux.cpp:54-141): ids 0-based, flip already applied. */
+typedef struct ps_joint {
+  int child_idx;
+  int parent_idx;
+  double offset_c[2]; /* parent_pos - child_pos in the child frame (pixels) */
+  double offset_p[2]; /* child_pos - parent_pos in the parent frame         */
+  double C[4];        /* 2x2 covariance, row major                          */
+  double rot_mean;    /* radians */
+  double rot_sigma;   /* radians */
+} ps_joint;
+
+/* PartConfig + ExpParam subset of one model (PartConfig.proto:1-59). */
+typedef struct ps_model_desc {
+  int num_parts;          /* P */
+  const int *is_detect;   /* [P] PartDef.is_detect (default 1)  */
+  const int *is_upright;  /* [P] PartDef.is_upright (default 0) */
+  const int *is_root;     /* [P] PartDef.is_root; the root is the one part with
+                             is_detect && is_root (objectdetect_findrot.cpp:781-786) */
+  int num_joints;         /* must be P - 1 */
+  const ps_joint *joints; /* default spatial model, used by instances with joints == NULL */
+  ps_exp_param exp_param;
+  int is_sparse;          /* bIsSparse of the upward pass; only the off-diagonal
+                             covariance path reads it */
+} ps_model_desc;
+
+typedef struct ps_model ps_model; /* opaque, immutable after creation        */
+typedef struct ps_plan ps_plan;   /* opaque: tables + scratch for one batch   */
+
+/* One image of a batch.  Offsets are in elements of the arrays passed to
+ * ps_plan_execute / ps_marginals, so instances of uneven size can be packed. */
+typedef struct ps_instance {
+  int height;                 /* H */
+  int width;                  /* W */
+  long long unary_offset;     /* start of this instance's [P][S][R][H][W] log unaries   */
+  long long marginal_offset;  /* start of its [P][S][R][H][W] log marginals (output)    */
+  long long root_offset;      /* start of its [S][H][W] root posterior (output)         */
+  const ps_joint *joints;     /* NULL: the model's joints; else num_joints joints with the
+                                 model's topology (image-conditioned spatial model)      */
+} ps_instance;
+
+/* MAP of one (instance, scale, part): the first flat argmax of the log
+ * marginal, objectdetect_findrot.cpp:255-277 (PartHyp, objectdetect.h:88-195). */
+typedef struct ps_map {
+  int rotidx;
+  int y;
+  int x;
+  float score;
+} ps_map;
+
+/* ------------------------------------------------------------------ models */
+int ps_model_create(const ps_model_desc *desc, ps_model **out);
+void ps_model_destroy(ps_model *model);
+
+/* ------------------------------------------------------------------- plans
+ * A plan fixes the batch (sizes, per-instance joints) and owns the device
+ * tables and the scratch of the sweep; executing it allocates nothing.
+ * Replaces the per-scale, per-message allocations of computeRootPosteriorRot
+ * (objectdetect_findrot.cpp:560-578, 339-340, 373). */
+int ps_plan_create(const ps_model *model, const ps_instance *instances, int n_instances, ps_plan **out);
+void ps_plan_destroy(ps_plan *plan);
+size_t ps_plan_scratch_bytes(const ps_plan *plan);
+/* Launch the whole sweep on `stream` (a hipStream_t, NULL = default stream).
+ * log_unary is read-only: the upright / border masks of
+ * objectdetect_findrot.cpp:509-551 are applied on the fly instead of being
+ * written into the caller's grids.  map_dev: [n_instances][S][P] ps_map in
+ * device memory.  Asynchronous.  root_posterior may be NULL. */
+int ps_plan_execute(ps_plan *plan, const float *log_unary, float *log_marginal, float *root_posterior,
+                    ps_map *map_dev, void *stream);
+/* Same, through a HIP graph captured on first use for this pointer set
+ * (re-captured when the pointers or the stream change). */
+int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, float *log_marginal, float *root_posterior,
+                          ps_map *map_dev, void *stream);
+
+/* Per-kernel timing of a plan's executions, for measurement: while enabled,
+ * every launch of ps_plan_execute is bracketed by HIP events on its stream.
+ * ps_plan_read_profile waits for the recorded launches and returns, per kernel
+ * class (0 reset, 1 leaf max, 2 row pass, 3 column pass, 4 root combine,
+ * 5 MAP decode), the summed milliseconds and the launch count since the last
+ * read; the counters then restart.  Not available with ps_plan_execute_graph. */
+int ps_plan_set_profiling(ps_plan *plan, int enable);
+int ps_plan_read_profile(ps_plan *plan, double ms[6], int launches[6]);
+
+/* ----------------------------------------------------- reference-shaped API
+ * computeRootPosteriorRot (objectdetect_findrot.cpp:470-727; objectdetect.h:269-274)
+ * for a batch: plan, execute, copy the MAP records to host memory
+ * (map_host: [n_instances][S][P]) and wait.  Device pointers as above. */
+int ps_marginals(const ps_model *model, const ps_instance *instances, int n_instances, const float *log_unary,
+                 float *log_marginal, float *root_posterior, ps_map *map_host, void *stream);
+
+/* computeRotJointMarginal (objectdetect_findrot.cpp:292-456; objectdetect.h:277-282):
+ * one message child[R][H][W] -> parent[R][H][W] on the device, with the
+ * model's exp_param.  Synchronous. */
+int ps_message(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
+               const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
+               double rot_sigma, double scale, void *stream);
+
+/* --------------------------------------------------------------- utilities */
+const char *ps_last_error_string(void);
+const char *ps_version(void);
+
+/* The integer plan of one message as the library computes it on the host:
+ * rotation shift delta, per-rotation warp offsets (forward indexed by source
+ * slice), the rotation taps (rot_len 0 when rot_sigma <= 0) and the half
+ * widths of the spatial filters for covariance C.  For tests. */
+int ps_debug_message_plan(const ps_exp_param *ep, int height, int width, const double offset_c[2],
+                          const double offset_p[2], const double C[4], double rot_mean, double rot_sigma,
+                          double scale, int *delta,
+                          int *fwd_ox, int *fwd_oy, int *inv_ox, int *inv_oy, int *rot_len, float *rot_taps,
+                          int *kx, int *ky);
+
+/* Seeded synthetic instance (the benchmark workload): log unaries
+ * N(-3, 0.5) plus 3 Gaussian bumps (amplitude 4, std 6 px) per part at seeded
+ * (x, y, rotation bin), and joints with offsets U(-40,40), sigma_x,y U[4,12],
+ * rot_sigma U[0.1,0.4], rot_mean U[-pi/6,pi/6] (0 when zero_rot_mean).
+ * tree: 0 = the 10-part person tree (root = part 4, torso).
+ * Host memory; unary: [P][S][R][H][W], joints: [P-1].  Deterministic in seed. */
+int ps_synth_joints(unsigned long long seed, int tree, int zero_rot_mean, ps_joint *joints, int *num_joints);
+int ps_synth_unary(unsigned long long seed, int num_parts, int num_scales, int num_rot, int height, int width,
+                   float *unary, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,766 @@
+// C ABI of the gfx950 pictorial-structure sweep (include/partapp_ps.h).
+//
+// A plan turns a batch of instances into device tables: one MsgJob per
+// (instance, scale, joint, direction), one RootJob per (instance, scale),
+// MaxJobs for the leaves and MapJobs for the MAP records.  Executing the plan
+// launches, in stream order,
+//   reset keys -> leaf maxima -> [rowpass, colpass] per upward level
+//   -> root combine -> [rowpass, colpass] per downward level -> MAP decode,
+// every launch covering all instances, scales and chains of its level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ps_internal.hpp"
+#include "ps_plan.hpp"
+
+namespace ps {
+extern thread_local std::string g_last_error;
+hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map, hipStream_t s);
+hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_rows, hipStream_t s);
+size_t rowpass_lds_bytes(int R, int W, int kx);
+hipError_t rowpass_set_lds_limit(size_t bytes);
+hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
+                          int R, int max_h, size_t lds_bytes, hipStream_t s);
+hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
+                          int R, int max_h, int max_w, hipStream_t s);
+hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
+hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
+}  // namespace ps
+
+using namespace ps;
+
+struct ps_model {
+  Model m;
+};
+
+namespace {
+
+struct Level {
+  int off = 0, count = 0, max_h = 0, max_w = 0;
+  size_t lds = 0;
+};
+
+int hip_error(hipError_t e, const char *what) {
+  return set_error(PS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr size_t kLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
+
+int check_device(int &device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(PS_ERR_NO_DEVICE, "no HIP device");
+  if (hipGetDevice(&device) != hipSuccess) return set_error(PS_ERR_NO_DEVICE, "hipGetDevice failed");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return set_error(PS_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(PS_ERR_NO_DEVICE, std::string("library is built for gfx950, device is ") + prop.gcnArchName);
+  return PS_OK;
+}
+
+GridRef gref(int buf, long long off) {
+  GridRef g;
+  g.buf = buf;
+  g.pad_ = 0;
+  g.off = off;
+  return g;
+}
+const GridRef kNone = {-1, 0, 0};
+
+}  // namespace
+
+struct ps_plan {
+  Model model;
+  int device = 0;
+  int n_inst = 0, S = 0, P = 0, R = 0, items = 0;
+  // device tables
+  void *tables = nullptr;
+  MsgJob *jobs = nullptr;
+  RootJob *roots = nullptr;
+  MaxJob *maxes = nullptr;
+  MapJob *maps = nullptr;
+  float *taps = nullptr;
+  int *shifts = nullptr;
+  float *scratch = nullptr;
+  size_t scratch_elems = 0;
+  unsigned *keys = nullptr;
+  int n_keys = 0;
+  unsigned long long *map_keys = nullptr;
+  int n_map = 0;
+  std::vector<Level> up, down;
+  int n_roots = 0, n_maxes = 0;
+  int max_rows = 0, max_hw = 0;
+  // graph cache
+  hipGraphExec_t gexec = nullptr;
+  const void *gptr[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipStream_t gstream = nullptr;
+  // profiling: event pairs per launch, pooled
+  bool profiling = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> event_class;  // class of each recorded pair
+  size_t n_recorded = 0;         // pairs recorded since the last read
+};
+
+namespace {
+constexpr int kProfileClasses = 6;
+
+// Record the start event of a launch of class `cls`; returns the pair index or -1.
+long profile_begin(ps_plan *pl, int cls, hipStream_t s) {
+  if (!pl || !pl->profiling) return -1;
+  size_t i = pl->n_recorded;
+  while (pl->events.size() < 2 * (i + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    pl->events.push_back(e);
+  }
+  if (pl->event_class.size() < i + 1) pl->event_class.resize(i + 1);
+  pl->event_class[i] = cls;
+  if (hipEventRecord(pl->events[2 * i], s) != hipSuccess) return -1;
+  pl->n_recorded = i + 1;
+  return (long)i;
+}
+void profile_end(ps_plan *pl, long i, hipStream_t s) {
+  if (i >= 0) (void)hipEventRecord(pl->events[2 * i + 1], s);
+}
+}  // namespace
+
+// ------------------------------------------------------------------ models
+
+extern "C" int ps_model_create(const ps_model_desc *desc, ps_model **out) {
+  if (!out) return set_error(PS_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  ps_model *m = new ps_model();
+  int st = build_model(desc, m->m);
+  if (st != PS_OK) {
+    delete m;
+    return st;
+  }
+  *out = m;
+  return PS_OK;
+}
+
+extern "C" void ps_model_destroy(ps_model *model) { delete model; }
+
+// ------------------------------------------------------------------- plans
+
+namespace {
+
+struct Builder {
+  const Model &m;
+  std::vector<MsgJob> jobs;
+  std::vector<RootJob> roots;
+  std::vector<MaxJob> maxes;
+  std::vector<MapJob> maps;
+  std::vector<float> taps;
+  std::vector<int> shifts;
+  explicit Builder(const Model &mm) : m(mm) {}
+
+  int add_taps(const std::vector<float> &f, int align) {
+    while (taps.size() % 16) taps.push_back(0.0f);
+    int off = (int)taps.size();
+    taps.insert(taps.end(), f.begin(), f.end());
+    while ((taps.size() - off) % align) taps.push_back(0.0f);
+    return off;
+  }
+  int add_message(const MessagePlan &mp, MsgJob &J) {
+    J.delta = mp.delta;
+    J.rot_mode = mp.rot_mode;
+    J.rot_len = (int)mp.rot_taps.size();
+    J.kx = mp.kx;
+    J.ky = mp.ky;
+    J.taps_r = add_taps(mp.rot_taps, 4);
+    J.taps_x = add_taps(mp.fx, 16);
+    J.taps_y = add_taps(mp.fy, 16);
+    J.shifts = (int)shifts.size();
+    shifts.insert(shifts.end(), mp.fwd_ox.begin(), mp.fwd_ox.end());
+    shifts.insert(shifts.end(), mp.fwd_oy.begin(), mp.fwd_oy.end());
+    shifts.insert(shifts.end(), mp.inv_ox.begin(), mp.inv_ox.end());
+    shifts.insert(shifts.end(), mp.inv_oy.begin(), mp.inv_oy.end());
+    return PS_OK;
+  }
+};
+
+UnaryMask part_mask(const Model &m, int p, int W) {
+  UnaryMask k;
+  k.rot_bits = m.is_upright[p] ? upright_masked_bits(m.ep) : 0ull;
+  k.strip = 0;
+  k.pad_ = 0;
+  if (p == m.tree.root && m.ep.strip_border_detections > 0) k.strip = (int)(m.ep.strip_border_detections * W);
+  return k;
+}
+
+}  // namespace
+
+extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instances, int n_instances, ps_plan **out) {
+  if (!out) return set_error(PS_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (!model || !instances || n_instances < 1) return set_error(PS_ERR_INVALID_ARG, "model, instances and n_instances >= 1 required");
+  const Model &m = model->m;
+  const Tree &t = m.tree;
+  const int P = m.P, S = (int)m.ep.num_scale_steps, R = (int)m.ep.num_rotation_steps;
+  const int K = (int)t.root_joints.size();
+  for (int i = 0; i < n_instances; ++i) {
+    const ps_instance &in = instances[i];
+    if (in.height < 1 || in.width < 1) return set_error(PS_ERR_INVALID_ARG, "instance sizes must be >= 1");
+    if ((long long)R * in.height * in.width >= (1ll << 31))
+      return set_error(PS_ERR_UNSUPPORTED, "R*H*W must be below 2^31");
+    if (in.unary_offset < 0 || in.marginal_offset < 0 || in.root_offset < 0)
+      return set_error(PS_ERR_INVALID_ARG, "negative offset");
+    if (in.joints) {
+      int st = check_joints(m, in.joints);
+      if (st != PS_OK) return st;
+    }
+  }
+  int device = 0;
+  int st = check_device(device);
+  if (st != PS_OK) return st;
+
+  // chains: nodes from the root child down to the leaf
+  std::vector<std::vector<int>> chains(K);
+  for (int k = 0; k < K; ++k) {
+    int p = m.joints[t.root_joints[k]].child_idx;
+    while (p >= 0) {
+      chains[k].push_back(p);
+      int cj = t.child_joint[p];
+      p = cj >= 0 ? m.joints[cj].child_idx : -1;
+    }
+  }
+
+  Builder B(m);
+  const int items = n_instances * S;
+  std::vector<std::vector<MsgJob>> up_jobs(t.max_height), dn_jobs(t.max_depth);
+  size_t scratch_off = 0;
+  int max_rows = 0, max_hw = 0;
+  for (int i = 0; i < n_instances; ++i) {
+    const ps_instance &in = instances[i];
+    const int H = in.height, W = in.width;
+    const long long n = (long long)R * H * W;
+    const ps_joint *J = in.joints ? in.joints : m.joints.data();
+    max_rows = std::max(max_rows, R * H);
+    max_hw = std::max(max_hw, H * W);
+    for (int s = 0; s < S; ++s) {
+      const int it = i * S + s;
+      const double scale = scale_from_index(m.ep, s);
+      auto unary = [&](int p) { return gref(kBufUnary, in.unary_offset + ((long long)p * S + s) * n); };
+      auto marg = [&](int p) { return gref(kBufMarginal, in.marginal_offset + ((long long)p * S + s) * n); };
+      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * n); };
+      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)(K + k) * n); };
+      auto up_key = [&](int p) { return it * 2 * P + p; };
+      auto dn_key = [&](int p) { return it * 2 * P + P + p; };
+      auto slot = [&](int p) { return it * P + p; };
+
+      // leaf maxima
+      for (int k = 0; k < K; ++k) {
+        int leaf = chains[k].back();
+        MaxJob mj;
+        mj.H = H;
+        mj.W = W;
+        mj.src_kind = m.is_detect[leaf] ? kSrcUnary : kSrcZero;
+        mj.key = up_key(leaf);
+        mj.src = unary(leaf);
+        mj.mask = part_mask(m, leaf, W);
+        B.maxes.push_back(mj);
+      }
+      // upward levels: the edge of chain k whose child has height l
+      for (int k = 0; k < K; ++k) {
+        const int len = (int)chains[k].size();
+        for (int d = len; d >= 1; --d) {
+          const int c = chains[k][d - 1];
+          const int jidx = t.parent_joint[c];
+          const ps_joint &jt = J[jidx];
+          const int p = m.joints[jidx].parent_idx;
+          MessagePlan mp;
+          st = build_message_plan(m.ep, H, W, jt.offset_c, jt.offset_p, jt.C, jt.rot_mean, jt.rot_sigma, scale, mp);
+          if (st != PS_OK) return set_error(st, "instance " + std::to_string(i) + ", joint " + std::to_string(jidx) + " (up): " + g_last_error);
+          MsgJob mj;
+          std::memset(&mj, 0, sizeof(mj));
+          mj.H = H;
+          mj.W = W;
+          B.add_message(mp, mj);
+          if (t.height[c] == 1) {
+            mj.src_kind = m.is_detect[c] ? kSrcUnary : kSrcZero;
+            mj.src = unary(c);
+            mj.src_mask = part_mask(m, c, W);
+          } else {
+            mj.src_kind = kSrcGrid;
+            mj.src = marg(c);
+          }
+          mj.src_key = up_key(c);
+          mj.tmp = tbuf(k);
+          mj.dst2 = kNone;
+          mj.dst_unary = kNone;
+          if (p == t.root) {
+            mj.dst_kind = kDstUpRoot;
+            mj.dst = mbuf(k);
+          } else {
+            mj.dst_kind = kDstUpNode;
+            mj.dst = marg(p);
+            mj.dst_unary = unary(p);
+            mj.dst_mask = part_mask(m, p, W);
+            mj.dst_add_unary = m.is_detect[p];
+            mj.dst_key = up_key(p);
+          }
+          up_jobs[t.height[c] - 1].push_back(mj);
+        }
+      }
+      // root combine
+      {
+        RootJob rj;
+        std::memset(&rj, 0, sizeof(rj));
+        rj.H = H;
+        rj.W = W;
+        rj.K = K;
+        rj.add_unary = m.is_detect[t.root];
+        for (int k = 0; k < K; ++k) {
+          rj.msg[k] = mbuf(k);
+          rj.fr_key[k] = dn_key(chains[k][0]);
+        }
+        rj.unary = unary(t.root);
+        rj.mask = part_mask(m, t.root, W);
+        rj.marginal = marg(t.root);
+        rj.map_slot = slot(t.root);
+        rj.root_post = gref(kBufRoot, in.root_offset + (long long)s * H * W);
+        if (m.is_upright[t.root]) {
+          int k1 = index_from_rot(m.ep, -1e-6), k2 = index_from_rot(m.ep, 1e-6);
+          if (k1 < 0 || k2 < 0 || k1 >= R || k2 >= R)
+            return set_error(PS_ERR_INVALID_ARG, "upright root needs rotation 0 inside the rotation range");
+          rj.valid_rot = (1ull << k1) | (1ull << k2);
+        } else {
+          rj.valid_rot = R == 64 ? ~0ull : ((1ull << R) - 1);
+        }
+        B.roots.push_back(rj);
+      }
+      // downward levels: the edge of chain k whose child has depth l
+      for (int k = 0; k < K; ++k) {
+        const int len = (int)chains[k].size();
+        for (int d = 1; d <= len; ++d) {
+          const int c = chains[k][d - 1];
+          const int jidx = t.parent_joint[c];
+          const ps_joint &jt = J[jidx];
+          MessagePlan mp;
+          // objectdetect_findrot.cpp:174-179, 224-229: offsets swapped, -rot_mean
+          st = build_message_plan(m.ep, H, W, jt.offset_p, jt.offset_c, jt.C, -jt.rot_mean, jt.rot_sigma, scale, mp);
+          if (st != PS_OK) return set_error(st, "instance " + std::to_string(i) + ", joint " + std::to_string(jidx) + " (down): " + g_last_error);
+          MsgJob mj;
+          std::memset(&mj, 0, sizeof(mj));
+          mj.H = H;
+          mj.W = W;
+          B.add_message(mp, mj);
+          mj.src_kind = kSrcGrid;
+          mj.src = mbuf(k);
+          mj.src_key = dn_key(c);
+          mj.tmp = tbuf(k);
+          mj.dst_kind = kDstDown;
+          mj.dst = marg(c);
+          mj.dst_unary = unary(c);
+          mj.dst_mask = part_mask(m, c, W);
+          const bool has_child = t.child_joint[c] >= 0;
+          mj.dst_base = has_child ? 0 : (m.is_detect[c] ? 1 : 2);
+          if (has_child) {
+            mj.dst2 = mbuf(k);
+            mj.dst2_key = dn_key(m.joints[t.child_joint[c]].child_idx);
+          } else {
+            mj.dst2 = kNone;
+          }
+          mj.map_slot = slot(c);
+          dn_jobs[d - 1].push_back(mj);
+        }
+      }
+      for (int p = 0; p < P; ++p) {
+        MapJob mj;
+        mj.H = H;
+        mj.W = W;
+        mj.slot = slot(p);
+        mj.out = it * P + p;
+        B.maps.push_back(mj);
+      }
+      scratch_off += (size_t)2 * K * n;
+    }
+  }
+
+  ps_plan *pl = new ps_plan();
+  pl->model = m;
+  pl->device = device;
+  pl->n_inst = n_instances;
+  pl->S = S;
+  pl->P = P;
+  pl->R = R;
+  pl->items = items;
+  pl->max_rows = max_rows;
+  pl->max_hw = max_hw;
+  auto add_levels = [&](std::vector<std::vector<MsgJob>> &lv, std::vector<Level> &dst) -> int {
+    for (auto &v : lv) {
+      Level L;
+      L.off = (int)B.jobs.size();
+      L.count = (int)v.size();
+      for (const MsgJob &j : v) {
+        L.max_h = std::max(L.max_h, j.H);
+        L.max_w = std::max(L.max_w, j.W);
+        L.lds = std::max(L.lds, rowpass_lds_bytes(R, j.W, j.kx));
+      }
+      if (L.lds > kLdsLimit)
+        return set_error(PS_ERR_UNSUPPORTED, "row pass needs " + std::to_string(L.lds) +
+                                                 " bytes of LDS (R * width too large for one workgroup)");
+      B.jobs.insert(B.jobs.end(), v.begin(), v.end());
+      dst.push_back(L);
+    }
+    return PS_OK;
+  };
+  st = add_levels(up_jobs, pl->up);
+  if (st == PS_OK) st = add_levels(dn_jobs, pl->down);
+  if (st != PS_OK) {
+    delete pl;
+    return st;
+  }
+  pl->n_roots = (int)B.roots.size();
+  pl->n_maxes = (int)B.maxes.size();
+  pl->n_keys = items * 2 * P;
+  pl->n_map = items * P;
+  pl->scratch_elems = scratch_off;
+
+  // one allocation for the tables, one for the scratch grids
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t o_jobs = 0;
+  size_t o_roots = align(o_jobs + B.jobs.size() * sizeof(MsgJob));
+  size_t o_maxes = align(o_roots + B.roots.size() * sizeof(RootJob));
+  size_t o_maps = align(o_maxes + B.maxes.size() * sizeof(MaxJob));
+  size_t o_taps = align(o_maps + B.maps.size() * sizeof(MapJob));
+  size_t o_shifts = align(o_taps + B.taps.size() * sizeof(float));
+  size_t o_keys = align(o_shifts + B.shifts.size() * sizeof(int));
+  size_t o_mkeys = align(o_keys + (size_t)pl->n_keys * sizeof(unsigned));
+  size_t total = align(o_mkeys + (size_t)pl->n_map * sizeof(unsigned long long));
+  hipError_t e = hipMalloc(&pl->tables, total);
+  if (e != hipSuccess) {
+    delete pl;
+    return set_error(PS_ERR_OUT_OF_MEMORY, "tables: " + std::string(hipGetErrorString(e)));
+  }
+  char *base = static_cast<char *>(pl->tables);
+  pl->jobs = reinterpret_cast<MsgJob *>(base + o_jobs);
+  pl->roots = reinterpret_cast<RootJob *>(base + o_roots);
+  pl->maxes = reinterpret_cast<MaxJob *>(base + o_maxes);
+  pl->maps = reinterpret_cast<MapJob *>(base + o_maps);
+  pl->taps = reinterpret_cast<float *>(base + o_taps);
+  pl->shifts = reinterpret_cast<int *>(base + o_shifts);
+  pl->keys = reinterpret_cast<unsigned *>(base + o_keys);
+  pl->map_keys = reinterpret_cast<unsigned long long *>(base + o_mkeys);
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data() + o_jobs, B.jobs.data(), B.jobs.size() * sizeof(MsgJob));
+  std::memcpy(host.data() + o_roots, B.roots.data(), B.roots.size() * sizeof(RootJob));
+  std::memcpy(host.data() + o_maxes, B.maxes.data(), B.maxes.size() * sizeof(MaxJob));
+  std::memcpy(host.data() + o_maps, B.maps.data(), B.maps.size() * sizeof(MapJob));
+  std::memcpy(host.data() + o_taps, B.taps.data(), B.taps.size() * sizeof(float));
+  std::memcpy(host.data() + o_shifts, B.shifts.data(), B.shifts.size() * sizeof(int));
+  e = hipMemcpy(pl->tables, host.data(), total, hipMemcpyHostToDevice);
+  if (e == hipSuccess && pl->scratch_elems) {
+    e = hipMalloc(&pl->scratch, pl->scratch_elems * sizeof(float));
+    if (e != hipSuccess) {
+      (void)hipFree(pl->tables);
+      delete pl;
+      return set_error(PS_ERR_OUT_OF_MEMORY, "scratch (" + std::to_string(pl->scratch_elems * 4) + " bytes): " + hipGetErrorString(e));
+    }
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(pl->tables);
+    delete pl;
+    return hip_error(e, "plan upload");
+  }
+  size_t max_lds = 0;
+  for (auto &L : pl->up) max_lds = std::max(max_lds, L.lds);
+  for (auto &L : pl->down) max_lds = std::max(max_lds, L.lds);
+  e = rowpass_set_lds_limit(max_lds);
+  if (e != hipSuccess) {
+    ps_plan_destroy(pl);
+    return hip_error(e, "hipFuncSetAttribute");
+  }
+  *out = pl;
+  return PS_OK;
+}
+
+extern "C" int ps_plan_set_profiling(ps_plan *plan, int enable) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  plan->profiling = enable != 0;
+  return PS_OK;
+}
+
+extern "C" int ps_plan_read_profile(ps_plan *plan, double ms[6], int launches[6]) {
+  if (!plan || !ms || !launches) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  for (int c = 0; c < kProfileClasses; ++c) {
+    ms[c] = 0.0;
+    launches[c] = 0;
+  }
+  for (size_t i = 0; i < plan->n_recorded; ++i) {
+    hipError_t e = hipEventSynchronize(plan->events[2 * i + 1]);
+    if (e != hipSuccess) return hip_error(e, "hipEventSynchronize");
+    float t = 0.0f;
+    e = hipEventElapsedTime(&t, plan->events[2 * i], plan->events[2 * i + 1]);
+    if (e != hipSuccess) return hip_error(e, "hipEventElapsedTime");
+    ms[plan->event_class[i]] += t;
+    launches[plan->event_class[i]] += 1;
+  }
+  plan->n_recorded = 0;
+  return PS_OK;
+}
+
+extern "C" void ps_plan_destroy(ps_plan *plan) {
+  if (!plan) return;
+  for (hipEvent_t e : plan->events) (void)hipEventDestroy(e);
+  if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
+  if (plan->scratch) (void)hipFree(plan->scratch);
+  if (plan->tables) (void)hipFree(plan->tables);
+  delete plan;
+}
+
+extern "C" size_t ps_plan_scratch_bytes(const ps_plan *plan) {
+  return plan ? plan->scratch_elems * sizeof(float) : 0;
+}
+
+static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, float *root_posterior, ps_map *map_dev,
+                   hipStream_t s, bool profile) {
+  Bases b;
+  b.unary = log_unary;
+  b.marginal = log_marginal;
+  b.scratch = pl->scratch;
+  b.root = root_posterior;
+  b.max_keys = pl->keys;
+  b.map_keys = pl->map_keys;
+  ps_plan *pp = profile ? pl : nullptr;
+  long ev = profile_begin(pp, 0, s);
+  hipError_t e = launch_reset(pl->keys, pl->n_keys, pl->map_keys, pl->n_map, s);
+  profile_end(pp, ev, s);
+  if (e == hipSuccess) {
+    ev = profile_begin(pp, 1, s);
+    e = launch_max(b, pl->maxes, pl->n_maxes, pl->R, pl->max_rows, s);
+    profile_end(pp, ev, s);
+  }
+  auto level = [&](const Level &L) {
+    long ev2 = profile_begin(pp, 2, s);
+    e = launch_rowpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.lds, s);
+    profile_end(pp, ev2, s);
+    if (e != hipSuccess) return;
+    ev2 = profile_begin(pp, 3, s);
+    e = launch_colpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, s);
+    profile_end(pp, ev2, s);
+  };
+  for (const Level &L : pl->up) {
+    if (e != hipSuccess) break;
+    level(L);
+  }
+  if (e == hipSuccess) {
+    ev = profile_begin(pp, 4, s);
+    e = launch_root(b, pl->roots, pl->n_roots, pl->R, pl->max_hw, s);
+    profile_end(pp, ev, s);
+  }
+  for (const Level &L : pl->down) {
+    if (e != hipSuccess) break;
+    level(L);
+  }
+  if (e == hipSuccess && map_dev) {
+    ev = profile_begin(pp, 5, s);
+    e = launch_map_decode(b, pl->maps, pl->n_map, map_dev, s);
+    profile_end(pp, ev, s);
+  }
+  if (e != hipSuccess) return hip_error(e, "kernel launch");
+  return PS_OK;
+}
+
+static int check_exec_args(ps_plan *plan, const float *log_unary, float *log_marginal) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  if (!log_unary || !log_marginal) return set_error(PS_ERR_INVALID_ARG, "log_unary and log_marginal are required");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != plan->device)
+    return set_error(PS_ERR_INVALID_ARG, "the plan belongs to another device");
+  return PS_OK;
+}
+
+extern "C" int ps_plan_execute(ps_plan *plan, const float *log_unary, float *log_marginal, float *root_posterior,
+                               ps_map *map_dev, void *stream) {
+  int st = check_exec_args(plan, log_unary, log_marginal);
+  if (st != PS_OK) return st;
+  return enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, static_cast<hipStream_t>(stream),
+                 plan->profiling);
+}
+
+extern "C" int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, float *log_marginal,
+                                     float *root_posterior, ps_map *map_dev, void *stream) {
+  int st = check_exec_args(plan, log_unary, log_marginal);
+  if (st != PS_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!s) return set_error(PS_ERR_INVALID_ARG, "graph execution needs a non-default stream");
+  const void *ptrs[4] = {log_unary, log_marginal, root_posterior, map_dev};
+  bool same = plan->gexec && plan->gstream == s;
+  for (int i = 0; i < 4 && same; ++i) same = plan->gptr[i] == ptrs[i];
+  if (!same) {
+    if (plan->gexec) {
+      (void)hipGraphExecDestroy(plan->gexec);
+      plan->gexec = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_error(e, "hipStreamBeginCapture");
+    st = enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, s, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (st != PS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      plan->gexec = nullptr;
+      return hip_error(e, "hipGraphInstantiate");
+    }
+    plan->gstream = s;
+    for (int i = 0; i < 4; ++i) plan->gptr[i] = ptrs[i];
+  }
+  hipError_t e = hipGraphLaunch(plan->gexec, s);
+  if (e != hipSuccess) return hip_error(e, "hipGraphLaunch");
+  return PS_OK;
+}
+
+// ----------------------------------------------------- reference-shaped API
+
+extern "C" int ps_marginals(const ps_model *model, const ps_instance *instances, int n_instances,
+                            const float *log_unary, float *log_marginal, float *root_posterior, ps_map *map_host,
+                            void *stream) {
+  ps_plan *pl = nullptr;
+  int st = ps_plan_create(model, instances, n_instances, &pl);
+  if (st != PS_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  ps_map *map_dev = nullptr;
+  if (map_host) {
+    hipError_t e = hipMalloc(&map_dev, sizeof(ps_map) * pl->n_map);
+    if (e != hipSuccess) {
+      ps_plan_destroy(pl);
+      return set_error(PS_ERR_OUT_OF_MEMORY, "map buffer");
+    }
+  }
+  st = ps_plan_execute(pl, log_unary, log_marginal, root_posterior, map_dev, stream);
+  if (st == PS_OK && map_host) {
+    hipError_t e = hipMemcpyAsync(map_host, map_dev, sizeof(ps_map) * pl->n_map, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) st = hip_error(e, "map copy");
+  }
+  if (st == PS_OK) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) st = hip_error(e, "hipStreamSynchronize");
+  }
+  if (map_dev) (void)hipFree(map_dev);
+  ps_plan_destroy(pl);
+  return st;
+}
+
+extern "C" int ps_message(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
+                          const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
+                          double rot_sigma, double scale, void *stream) {
+  if (!model || !child_dev || !parent_dev || !offset_c || !offset_p || !C)
+    return set_error(PS_ERR_INVALID_ARG, "null argument");
+  if (height < 1 || width < 1) return set_error(PS_ERR_INVALID_ARG, "sizes must be >= 1");
+  const Model &m = model->m;
+  const int R = (int)m.ep.num_rotation_steps;
+  int device = 0;
+  int st = check_device(device);
+  if (st != PS_OK) return st;
+  MessagePlan mp;
+  st = build_message_plan(m.ep, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, mp);
+  if (st != PS_OK) return st;
+  size_t lds = rowpass_lds_bytes(R, width, mp.kx);
+  if (lds > kLdsLimit) return set_error(PS_ERR_UNSUPPORTED, "row pass LDS exceeds one workgroup");
+  Builder B(m);
+  MsgJob mj;
+  std::memset(&mj, 0, sizeof(mj));
+  mj.H = height;
+  mj.W = width;
+  B.add_message(mp, mj);
+  mj.src_kind = kSrcUnary;  // a plain grid, read through the (empty) mask path
+  mj.src = gref(kBufUnary, 0);
+  mj.src_key = 0;
+  mj.tmp = gref(kBufScratch, 0);
+  mj.dst_kind = kDstUpRoot;
+  mj.dst = gref(kBufMarginal, 0);
+  mj.dst_unary = kNone;
+  mj.dst2 = kNone;
+  MaxJob xj;
+  std::memset(&xj, 0, sizeof(xj));
+  xj.H = height;
+  xj.W = width;
+  xj.src_kind = kSrcUnary;
+  xj.key = 0;
+  xj.src = gref(kBufUnary, 0);
+
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t o_job = 0, o_max = align(sizeof(MsgJob)), o_taps = align(o_max + sizeof(MaxJob));
+  size_t o_sh = align(o_taps + B.taps.size() * sizeof(float));
+  size_t o_key = align(o_sh + B.shifts.size() * sizeof(int));
+  size_t o_mk = align(o_key + sizeof(unsigned));
+  size_t o_T = align(o_mk + sizeof(unsigned long long));
+  size_t total = o_T + sizeof(float) * (size_t)R * height * width;
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, total);
+  if (e != hipSuccess) return set_error(PS_ERR_OUT_OF_MEMORY, "message scratch");
+  std::vector<char> host(o_T, 0);
+  std::memcpy(host.data() + o_job, &mj, sizeof(mj));
+  std::memcpy(host.data() + o_max, &xj, sizeof(xj));
+  std::memcpy(host.data() + o_taps, B.taps.data(), B.taps.size() * sizeof(float));
+  std::memcpy(host.data() + o_sh, B.shifts.data(), B.shifts.size() * sizeof(int));
+  char *base = static_cast<char *>(mem);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  e = hipMemcpyAsync(mem, host.data(), o_T, hipMemcpyHostToDevice, s);
+  Bases b;
+  b.unary = child_dev;
+  b.marginal = parent_dev;
+  b.scratch = reinterpret_cast<float *>(base + o_T);
+  b.root = nullptr;
+  b.max_keys = reinterpret_cast<unsigned *>(base + o_key);
+  b.map_keys = reinterpret_cast<unsigned long long *>(base + o_mk);
+  const MsgJob *jd = reinterpret_cast<const MsgJob *>(base + o_job);
+  const float *td = reinterpret_cast<const float *>(base + o_taps);
+  const int *sd = reinterpret_cast<const int *>(base + o_sh);
+  if (e == hipSuccess) e = rowpass_set_lds_limit(lds);
+  if (e == hipSuccess) e = launch_reset(b.max_keys, 1, b.map_keys, 1, s);
+  if (e == hipSuccess) e = launch_max(b, reinterpret_cast<const MaxJob *>(base + o_max), 1, R, R * height, s);
+  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, R, height, lds, s);
+  if (e == hipSuccess) e = launch_colpass(b, jd, 1, td, sd, R, height, width, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(mem);
+  if (e != hipSuccess) return hip_error(e, "message");
+  return PS_OK;
+}
+
+// --------------------------------------------------------------- utilities
+
+extern "C" const char *ps_last_error_string(void) { return g_last_error.c_str(); }
+
+extern "C" const char *ps_version(void) { return "partapp_ps 0.1 (gfx950)"; }
+
+extern "C" int ps_debug_message_plan(const ps_exp_param *ep, int height, int width, const double offset_c[2],
+                                     const double offset_p[2], const double C[4], double rot_mean,
+                                     double rot_sigma, double scale,
+                                     int *delta, int *fwd_ox, int *fwd_oy, int *inv_ox, int *inv_oy, int *rot_len,
+                                     float *rot_taps, int *kx, int *ky) {
+  if (!ep || !offset_c || !offset_p || !C || !delta || !fwd_ox || !fwd_oy || !inv_ox || !inv_oy || !rot_len || !kx ||
+      !ky)
+    return set_error(PS_ERR_INVALID_ARG, "null argument");
+  if (ep->num_rotation_steps < 1 || ep->num_rotation_steps > (unsigned)kMaxRot)
+    return set_error(PS_ERR_UNSUPPORTED, "num_rotation_steps must be in [1, 64]");
+  MessagePlan mp;
+  int st = build_message_plan(*ep, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, mp);
+  if (st != PS_OK) return st;
+  const int R = (int)ep->num_rotation_steps;
+  *delta = mp.delta;
+  for (int r = 0; r < R; ++r) {
+    fwd_ox[r] = mp.fwd_ox[r];
+    fwd_oy[r] = mp.fwd_oy[r];
+    inv_ox[r] = mp.inv_ox[r];
+    inv_oy[r] = mp.inv_oy[r];
+  }
+  *rot_len = mp.rot_mode == kRotFilter ? (int)mp.rot_taps.size() : 0;
+  if (rot_taps)
+    for (size_t i = 0; i < mp.rot_taps.size(); ++i) rot_taps[i] = mp.rot_taps[i];
+  *kx = mp.kx;
+  *ky = mp.ky;
+  return PS_OK;
+}

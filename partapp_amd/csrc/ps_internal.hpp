@@ -1,0 +1,111 @@
+// Internal structures shared by the host planner (ps_plan.cpp, ps_api.hip)
+// and the gfx950 kernels (ps_kernels.hip).  Plain structs only: they are
+// copied to device memory as tables.
+#pragma once
+
+#include <stdint.h>
+
+namespace ps {
+
+constexpr float kLogZero = -1e6f;  // LOG_ZERO, boost_math.h:23
+constexpr int kMaxRot = 64;        // rotation masks are 64-bit words
+constexpr int kMaxRootChildren = 16;
+constexpr int kMaxTaps = 1000;     // multi_array_filter.hpp:243-244
+
+// Which base pointer an offset is relative to.
+enum BufId : int { kBufUnary = 0, kBufMarginal = 1, kBufScratch = 2, kBufRoot = 3 };
+
+struct GridRef {
+  int buf;  // BufId, or -1 for "none"
+  int pad_;
+  long long off;  // element offset into that buffer
+};
+
+// The base pointers of one execution, passed by value to every kernel.
+struct Bases {
+  const float *unary;
+  float *marginal;
+  float *scratch;
+  float *root;
+  unsigned *max_keys;            // order-preserving keys of grid maxima
+  unsigned long long *map_keys;  // (score, first index) argmax keys
+};
+
+// Upright / border masking of one part's unaries
+// (objectdetect_findrot.cpp:509-551), applied on read.
+struct UnaryMask {
+  unsigned long long rot_bits;  // bit r set: rotation r reads LOG_ZERO
+  int strip;                    // columns [0,strip) and [W-strip,W) read LOG_ZERO
+  int pad_;
+};
+
+enum SrcKind : int { kSrcGrid = 0, kSrcUnary = 1, kSrcZero = 2 };
+enum DstKind : int { kDstUpNode = 0, kDstUpRoot = 1, kDstDown = 2 };
+enum RotMode : int { kRotFilter = 0, kRotCopy = 1, kRotZero = 2 };
+
+// One message (instance, scale, joint, direction) of the sweep.  A message is
+// computeRotJointMarginal (objectdetect_findrot.cpp:292-456) followed by the
+// accumulation its caller performs on the result.
+struct MsgJob {
+  int H, W;
+  int delta;     // rotation shift: output slice = source slice + delta
+  int rot_mode;  // RotMode
+  int rot_len;   // taps of the circular rotation filter
+  int kx, ky;    // half widths of the x and y filters
+  int taps_r;    // offsets into the tap pool (floats); x/y tap runs are padded
+  int taps_x;    //   with zeros to a multiple of 16
+  int taps_y;
+  int shifts;    // offset into the shift pool: fwd_ox[R] fwd_oy[R] inv_ox[R] inv_oy[R]
+  int src_kind;  // SrcKind
+  int src_key;   // index of the source's max key
+  GridRef src;   // [R][H][W]
+  UnaryMask src_mask;
+  GridRef tmp;   // row-pass output [R][H][W]
+  // destination
+  int dst_kind;       // DstKind
+  int dst_key;        // kDstUpNode: key of the parent belief (next source)
+  GridRef dst;        // UpNode: parent belief; UpRoot: message slot; Down: child marginal
+  GridRef dst_unary;  // unary of the part written (UpNode: parent, Down: child)
+  UnaryMask dst_mask;
+  int dst_add_unary;  // UpNode: parent is_detect
+  int dst_base;       // Down: 0 = dst holds the upward belief, 1 = masked unary, 2 = zero
+  GridRef dst2;       // Down with a grandchild: unary + message (next source), else buf -1
+  int dst2_key;
+  int map_slot;       // Down: argmax key of the child marginal
+};
+
+// Root of one (instance, scale): sums the upward messages into the root
+// belief, forms the "all but one child" root messages
+// (objectdetect_findrot.cpp:637-649, 157-170) and the root posterior (694-726).
+struct RootJob {
+  int H, W;
+  int K;  // root children, in joint order
+  int add_unary;
+  GridRef msg[kMaxRootChildren];  // upward messages; overwritten with log_from_root
+  int fr_key[kMaxRootChildren];
+  GridRef unary;
+  UnaryMask mask;
+  GridRef marginal;  // root marginal (= belief after the upward pass)
+  int map_slot;
+  int pad_;
+  GridRef root_post;  // [H][W] of this scale, buf -1 when not wanted
+  unsigned long long valid_rot;  // rotations summed by the root posterior
+};
+
+// Max of a leaf source grid.
+struct MaxJob {
+  int H, W;
+  int src_kind;
+  int key;
+  GridRef src;
+  UnaryMask mask;
+};
+
+// Decoding of one argmax key into a ps_map record.
+struct MapJob {
+  int H, W;
+  int slot;
+  int out;  // index into the ps_map array
+};
+
+}  // namespace ps

@@ -1,0 +1,504 @@
+// gfx950 kernels of the pictorial-structure sweep.
+//
+// One message of the sum-product sweep (computeRotJointMarginal,
+// objectdetect_findrot.cpp:292-456) is two launches over a whole level of
+// messages (every instance, scale and chain of the level at once):
+//
+//   k_rowpass  one block per (message, row y): gathers the forward-warped
+//              child belief of all R rotation slices (integer shift per
+//              slice), exp(b - M), circular rotation filter, then the
+//              truncated x filter from an LDS row with zero halo.  Writes the
+//              row-filtered grid T.
+//   k_colpass  one block per (message, rotation, 64x64 tile): truncated y
+//              filter of T with a 16-row register window per lane, log,
+//              +M, the inverse warp folded into the tile origin, and the
+//              caller's accumulation (parent belief, root message slot, or
+//              marginal + next downward source) fused as the epilogue,
+//              together with the max / argmax reductions the next step needs.
+//
+// Arithmetic follows the reference order: every dot product accumulates its
+// taps in ascending order with a fused multiply-add starting from 0, exactly
+// like a sequential sdot; sums of grids are added in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include "ps_internal.hpp"
+#include "../../include/partapp_ps.h"
+
+namespace ps {
+
+// ----------------------------------------------------------------- helpers
+
+__device__ __forceinline__ unsigned f2key(float f) {
+  unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(unsigned k) {
+  unsigned u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+// argmax key: larger score wins, then the smaller flat index (the first
+// maximum of a C-order scan, objectdetect_findrot.cpp:261-266)
+__device__ __forceinline__ unsigned long long amax_key(float v, unsigned idx) {
+  return ((unsigned long long)f2key(v) << 32) | (unsigned long long)(0xffffffffu - idx);
+}
+
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Base pointer of a GridRef.  Written as selects on the (uniform) buffer id
+// so the kernel argument block is never copied to scratch for indexing.
+__device__ __forceinline__ float *base_of(const Bases &b, int buf) {
+  float *p = buf == kBufMarginal ? b.marginal : b.scratch;
+  p = buf == kBufUnary ? const_cast<float *>(b.unary) : p;
+  p = buf == kBufRoot ? b.root : p;
+  return p;
+}
+__device__ __forceinline__ const float *cptr(const Bases &b, const GridRef &g) { return base_of(b, g.buf) + g.off; }
+__device__ __forceinline__ float *mptr(const Bases &b, const GridRef &g) { return base_of(b, g.buf) + g.off; }
+
+__device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) {
+  return ((m.rot_bits >> r) & 1ull) || x < m.strip || x >= W - m.strip;
+}
+
+// log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166)
+__device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
+
+// --------------------------------------------------------------- k_reset
+
+__global__ void k_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_keys) keys[i] = 0u;
+  if (i < n_map) map_keys[i] = 0ull;
+}
+
+// ------------------------------------------------------------------ k_max
+// Max of a leaf's source grid (getMinMax, multi_array_op.hpp:60-77), one
+// block per group of rows.
+
+__global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__ jobs, int R, int rows_per_block) {
+  const MaxJob &J = jobs[blockIdx.y];
+  const int H = J.H, W = J.W;
+  const int row0 = blockIdx.x * rows_per_block;
+  unsigned km = 0u;
+  if (J.src_kind == kSrcZero) {
+    km = f2key(0.0f);
+  } else {
+    const float *src = cptr(b, J.src);
+    for (int row = row0; row < min(row0 + rows_per_block, R * H); ++row) {
+      const int r = row / H;
+      const bool rot_masked = (J.mask.rot_bits >> r) & 1ull;
+      const float *p = src + (long long)row * W;
+      for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        float v = (rot_masked || x < J.mask.strip || x >= W - J.mask.strip) ? kLogZero : p[x];
+        km = max(km, f2key(v));
+      }
+    }
+  }
+  km = wave_max_u32(km);
+  if ((threadIdx.x & 63) == 0 && km != 0u) atomicMax(b.max_keys + J.key, km);
+}
+
+// -------------------------------------------------------------- k_rowpass
+// LDS: E[R][Wa] (exp of the warped child) and P[R][Pw] (rotation-filtered
+// rows with kx zeros on the left and a zero tail), Pw a multiple of 4.
+
+struct RowGeom {
+  int Wa, Wq, nchx, Pw;
+};
+__host__ __device__ __forceinline__ RowGeom row_geom(int W, int kx) {
+  RowGeom g;
+  g.Wa = (W + 3) & ~3;
+  g.Wq = (W + 3) >> 2;
+  g.nchx = (2 * kx + 1 + 3) >> 2;
+  g.Pw = 4 * (g.Wq + g.nchx + 1);
+  return g;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_rowpass(Bases b, const MsgJob *__restrict__ jobs,
+                                                const float *__restrict__ taps, const int *__restrict__ shifts,
+                                                int R) {
+  extern __shared__ float4 lds4[];
+  float *lds = reinterpret_cast<float *>(lds4);
+  const MsgJob &J = jobs[blockIdx.y];
+  const int y = blockIdx.x;
+  const int H = J.H, W = J.W;
+  if (y >= H) return;
+  const int kx = J.kx;
+  const RowGeom g = row_geom(W, kx);
+  float *E = lds;
+  float *P = lds + R * g.Wa;
+  const int tid = threadIdx.x;
+
+  const float M = key2f(b.max_keys[J.src_key]);
+  const float negM = -M;
+  const float e_fill = expf(kLogZero + negM);  // exp(LOG_ZERO - M) of the warp's fill value
+  const float *src = cptr(b, J.src);
+  const long long HW = (long long)H * W;
+  const int *fox = shifts + J.shifts;
+  const int *foy = fox + R;
+
+  // zero the halo columns of P
+  for (int i = tid; i < R * (kx + (g.Pw - kx - W)); i += NT) {
+    int r = i / (g.Pw - W);
+    int c = i - r * (g.Pw - W);
+    int p = c < kx ? c : c + W;
+    P[r * g.Pw + p] = 0.0f;
+  }
+
+  const int n_rot = J.rot_len >> 1;
+  const float *fr = taps + J.taps_r;
+  for (int x = tid; x < W; x += NT) {
+    // forward warp + exp (objectdetect_findrot.cpp:342-365)
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = ro - J.delta;
+      float e = e_fill;
+      if (rs >= 0 && rs < R) {
+        const int yy = y + foy[rs];
+        const int xx = x + fox[rs];
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+          float v;
+          if (J.src_kind == kSrcZero) v = 0.0f;
+          else if (J.src_kind == kSrcUnary && masked(J.src_mask, rs, xx, W)) v = kLogZero;
+          else v = src[rs * HW + (long long)yy * W + xx];
+          e = expf(v + negM);
+        }
+      }
+      E[ro * g.Wa + x] = e;
+    }
+    // circular rotation filter (objectdetect_findrot.cpp:377-420,
+    // multi_array_filter.hpp:115-155); the column was written by this thread
+    for (int ro = 0; ro < R; ++ro) {
+      float acc;
+      if (J.rot_mode == kRotFilter) {
+        acc = 0.0f;
+        for (int j = 0; j < J.rot_len; ++j) {
+          int rr = ro - n_rot + j;
+          rr = rr < 0 ? rr + R : (rr >= R ? rr - R : rr);
+          acc = fmaf(E[rr * g.Wa + x], fr[j], acc);
+        }
+      } else if (J.rot_mode == kRotCopy) {
+        acc = E[ro * g.Wa + x];
+      } else {
+        acc = 0.0f;
+      }
+      P[ro * g.Pw + kx + x] = acc;
+    }
+  }
+  __syncthreads();
+
+  // truncated x filter (multi_array_filter.hpp:277-288): four outputs per
+  // lane, taps in chunks of four, a float4 window sliding along the row
+  const float4 *fx4 = reinterpret_cast<const float4 *>(taps + J.taps_x);
+  float *T = mptr(b, J.tmp);
+  const int n_items = R * g.Wq;
+  const float inv_wq = 1.0f / (float)g.Wq;
+  for (int item = tid; item < n_items; item += NT) {
+    int ro = (int)((float)item * inv_wq);
+    if (ro * g.Wq > item) --ro;
+    if ((ro + 1) * g.Wq <= item) ++ro;
+    const int q = item - ro * g.Wq;
+    const float4 *P4 = reinterpret_cast<const float4 *>(P + ro * g.Pw);
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    float4 lo = P4[q];
+    for (int c = 0; c < g.nchx; ++c) {
+      const float4 hi = P4[q + c + 1];
+      const float4 f = fx4[c];
+      a0 = fmaf(lo.x, f.x, a0); a1 = fmaf(lo.y, f.x, a1); a2 = fmaf(lo.z, f.x, a2); a3 = fmaf(lo.w, f.x, a3);
+      a0 = fmaf(lo.y, f.y, a0); a1 = fmaf(lo.z, f.y, a1); a2 = fmaf(lo.w, f.y, a2); a3 = fmaf(hi.x, f.y, a3);
+      a0 = fmaf(lo.z, f.z, a0); a1 = fmaf(lo.w, f.z, a1); a2 = fmaf(hi.x, f.z, a2); a3 = fmaf(hi.y, f.z, a3);
+      a0 = fmaf(lo.w, f.w, a0); a1 = fmaf(hi.x, f.w, a1); a2 = fmaf(hi.y, f.w, a2); a3 = fmaf(hi.z, f.w, a3);
+      lo = hi;
+    }
+    float *trow = T + ro * HW + (long long)y * W;
+    const int x0 = 4 * q;
+    if ((W & 3) == 0) {
+      *reinterpret_cast<float4 *>(trow + x0) = make_float4(a0, a1, a2, a3);
+    } else {
+      if (x0 + 0 < W) trow[x0 + 0] = a0;
+      if (x0 + 1 < W) trow[x0 + 1] = a1;
+      if (x0 + 2 < W) trow[x0 + 2] = a2;
+      if (x0 + 3 < W) trow[x0 + 3] = a3;
+    }
+  }
+}
+
+// -------------------------------------------------------------- k_colpass
+
+constexpr int kColTileX = 64;   // lanes of a wave: one column each
+constexpr int kColRows = 16;    // outputs per lane
+constexpr int kColGroups = 4;   // waves per block
+constexpr int kColTileY = kColRows * kColGroups;
+
+__global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restrict__ jobs,
+                                                 const float *__restrict__ taps, const int *__restrict__ shifts,
+                                                 int R, int ytiles) {
+  const MsgJob &J = jobs[blockIdx.z];
+  const int H = J.H, W = J.W;
+  const int r = blockIdx.y / ytiles;
+  const int yt = blockIdx.y - r * ytiles;
+  const int lane = threadIdx.x & 63;
+  const int x = blockIdx.x * kColTileX + lane;
+  const int y0 = yt * kColTileY + (threadIdx.x >> 6) * kColRows;
+  // whole waves outside the grid leave; the others keep every lane for the
+  // wave reductions
+  if (blockIdx.x * kColTileX >= W || y0 >= H) return;
+  const bool xin = x < W;
+
+  const int *sh = shifts + J.shifts;
+  const int ox = sh[2 * R + r];
+  const int oy = sh[3 * R + r];
+  const int xs = x + ox;
+  const bool col_ok = xin && xs >= 0 && xs < W;
+  const long long HW = (long long)H * W;
+  const float *T = cptr(b, J.tmp) + r * HW + (col_ok ? xs : 0);
+  const int ky = J.ky, L = 2 * ky + 1;
+  const int q0 = y0 + oy - ky;  // source row of tap 0 for output 0
+  const float *fy = taps + J.taps_y;
+
+  auto load = [&](int q) -> float {
+    const int row = q0 + q;
+    return (col_ok && row >= 0 && row < H) ? T[(long long)row * W] : 0.0f;
+  };
+
+  float buf[kColRows], acc[kColRows];
+#pragma unroll
+  for (int i = 0; i < kColRows; ++i) {
+    buf[i] = load(i);
+    acc[i] = 0.0f;
+  }
+  // tap t of output i reads source row i + t, kept in slot (i + t) % 16
+  for (int t0 = 0; t0 < L; t0 += kColRows) {
+#pragma unroll
+    for (int u = 0; u < kColRows; ++u) {
+      const int t = t0 + u;
+      if (t < L) {
+        const float f = fy[t];
+#pragma unroll
+        for (int i = 0; i < kColRows; ++i) acc[i] = fmaf(buf[(i + u) & (kColRows - 1)], f, acc[i]);
+        buf[u] = load(t + kColRows);
+      }
+    }
+  }
+
+  const float M = key2f(b.max_keys[J.src_key]);
+  unsigned kmax1 = 0u, kmax2 = 0u;
+  unsigned long long amax = 0ull;
+  float *dst = mptr(b, J.dst);
+  const float *dun = J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : nullptr;
+  float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
+  const bool rmask = (J.dst_mask.rot_bits >> r) & 1ull;
+  const bool xmask = rmask || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
+
+#pragma unroll
+  for (int i = 0; i < kColRows; ++i) {
+    const int y = y0 + i;
+    if (!xin || y >= H) continue;
+    const int ysrc = y + oy;
+    // inverse warp (objectdetect_findrot.cpp:437-448): outside -> LOG_ZERO
+    const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(acc[i]) + M : kLogZero;
+    const long long e = r * HW + (long long)y * W + x;
+    if (J.dst_kind == kDstUpRoot) {
+      dst[e] = msg;
+    } else if (J.dst_kind == kDstUpNode) {
+      // belief of a chain node: 0 + message (+ unary), :637, :652-654
+      float v = msg;
+      if (J.dst_add_unary) v = v + (xmask ? kLogZero : dun[e]);
+      dst[e] = v;
+      kmax1 = max(kmax1, f2key(v));
+    } else {
+      // marginal = belief + downward message (:201); next source = unary + message (:220-222)
+      const float u = xmask ? kLogZero : dun[e];
+      const float base = J.dst_base == 0 ? dst[e] : (J.dst_base == 1 ? u : 0.0f);
+      const float mg = base + msg;
+      dst[e] = mg;
+      const unsigned long long k = amax_key(mg, (unsigned)e);
+      amax = k > amax ? k : amax;
+      if (dst2) {
+        const float v2 = u + msg;
+        dst2[e] = v2;
+        kmax2 = max(kmax2, f2key(v2));
+      }
+    }
+  }
+  if (J.dst_kind == kDstUpNode) {
+    kmax1 = wave_max_u32(kmax1);
+    if (lane == 0 && kmax1) atomicMax(b.max_keys + J.dst_key, kmax1);
+  } else if (J.dst_kind == kDstDown) {
+    amax = wave_max_u64(amax);
+    if (lane == 0 && amax) atomicMax(b.map_keys + J.map_slot, amax);
+    if (dst2) {
+      kmax2 = wave_max_u32(kmax2);
+      if (lane == 0 && kmax2) atomicMax(b.max_keys + J.dst2_key, kmax2);
+    }
+  }
+}
+
+// ----------------------------------------------------------------- k_root
+
+__global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict__ jobs, int R) {
+  const RootJob &J = jobs[blockIdx.y];
+  const int H = J.H, W = J.W;
+  const int HW = H * W;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x * blockDim.x >= HW) return;
+  const bool ok = pix < HW;
+  const int x = ok ? pix % W : 0;
+  const int K = J.K;
+  const float *un = cptr(b, J.unary);
+  float *marg = mptr(b, J.marginal);
+
+  unsigned kmax[kMaxRootChildren];
+#pragma unroll
+  for (int k = 0; k < kMaxRootChildren; ++k) kmax[k] = 0u;
+  unsigned long long amax = 0ull;
+  float sum = 0.0f;
+  bool first = true;
+
+  for (int r = 0; r < R; ++r) {
+    if (!ok) break;
+    const long long e = (long long)r * HW + pix;
+    float m[kMaxRootChildren];
+#pragma unroll
+    for (int k = 0; k < kMaxRootChildren; ++k)
+      if (k < K) m[k] = cptr(b, J.msg[k])[e];
+    const float u = masked(J.mask, r, x, W) ? kLogZero : un[e];
+    // root belief: messages in joint order (:637), then the unary (:652-654)
+    float bel = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kMaxRootChildren; ++k)
+      if (k < K) bel = bel + m[k];
+    if (J.add_unary) bel = bel + u;
+    marg[e] = bel;
+    const unsigned long long ak = amax_key(bel, (unsigned)e);
+    amax = ak > amax ? ak : amax;
+    // log_from_root of child i: the other messages in order (:641-649), then
+    // the root unary (:169)
+#pragma unroll
+    for (int i = 0; i < kMaxRootChildren; ++i) {
+      if (i < K) {
+        float fr = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kMaxRootChildren; ++k)
+          if (k < K && k != i) fr = fr + m[k];
+        fr = fr + u;
+        mptr(b, J.msg[i])[e] = fr;
+        kmax[i] = max(kmax[i], f2key(fr));
+      }
+    }
+    // root posterior over the valid rotations (:713-726), no max shift
+    if ((J.valid_rot >> r) & 1ull) {
+      const float ev = expf(bel);
+      sum = first ? ev : sum + ev;
+      first = false;
+    }
+  }
+  if (ok && J.root_post.buf >= 0 && b.root) mptr(b, J.root_post)[pix] = log0(sum);
+
+  const int lane = threadIdx.x & 63;
+  amax = wave_max_u64(amax);
+  if (lane == 0 && amax) atomicMax(b.map_keys + J.map_slot, amax);
+#pragma unroll
+  for (int k = 0; k < kMaxRootChildren; ++k) {
+    if (k < K) {
+      unsigned v = wave_max_u32(kmax[k]);
+      if (lane == 0 && v) atomicMax(b.max_keys + J.fr_key[k], v);
+    }
+  }
+}
+
+// ------------------------------------------------------------ k_map_decode
+
+__global__ void k_map_decode(Bases b, const MapJob *__restrict__ jobs, int n, ps_map *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const MapJob J = jobs[i];
+  const unsigned long long k = b.map_keys[J.slot];
+  ps_map m;
+  if (k == 0ull) {
+    m.rotidx = -1;
+    m.y = -1;
+    m.x = -1;
+    m.score = -INFINITY;
+  } else {
+    const unsigned idx = 0xffffffffu - (unsigned)(k & 0xffffffffull);
+    const unsigned hw = (unsigned)J.H * (unsigned)J.W;
+    m.rotidx = (int)(idx / hw);
+    const unsigned rem = idx - (unsigned)m.rotidx * hw;
+    m.y = (int)(rem / (unsigned)J.W);
+    m.x = (int)(rem - (unsigned)m.y * (unsigned)J.W);
+    m.score = key2f((unsigned)(k >> 32));
+  }
+  out[J.out] = m;
+}
+
+// ------------------------------------------------------------ host launchers
+
+hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map, hipStream_t s) {
+  int n = n_keys > n_map ? n_keys : n_map;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, keys, n_keys, map_keys, n_map);
+  return hipGetLastError();
+}
+
+hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_rows, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  const int rows_per_block = 8;
+  dim3 grid((max_rows + rows_per_block - 1) / rows_per_block, n_jobs);
+  hipLaunchKernelGGL(k_max, grid, dim3(256), 0, s, b, jobs, R, rows_per_block);
+  return hipGetLastError();
+}
+
+size_t rowpass_lds_bytes(int R, int W, int kx) {
+  RowGeom g = row_geom(W, kx);
+  return sizeof(float) * ((size_t)R * g.Wa + (size_t)R * g.Pw);
+}
+
+hipError_t rowpass_set_lds_limit(size_t bytes) {
+  if (bytes <= 64 * 1024) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rowpass<256>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
+                          int R, int max_h, size_t lds_bytes, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  dim3 grid(max_h, n_jobs);
+  hipLaunchKernelGGL(k_rowpass<256>, grid, dim3(256), lds_bytes, s, b, jobs, taps, shifts, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
+                          int R, int max_h, int max_w, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  const int ytiles = (max_h + kColTileY - 1) / kColTileY;
+  dim3 grid((max_w + kColTileX - 1) / kColTileX, R * ytiles, n_jobs);
+  hipLaunchKernelGGL(k_colpass, grid, dim3(256), 0, s, b, jobs, taps, shifts, R, ytiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s) {
+  if (n_jobs <= 0) return hipSuccess;
+  dim3 grid((max_hw + 255) / 256, n_jobs);
+  hipLaunchKernelGGL(k_root, grid, dim3(256), 0, s, b, jobs, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_map_decode, dim3((n + 255) / 256), dim3(256), 0, s, b, jobs, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace ps

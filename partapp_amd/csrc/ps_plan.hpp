@@ -1,0 +1,64 @@
+// Host-side planning of the sweep: model validation, the level schedule of
+// the tree, and the integer plan + filter taps of every message.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/partapp_ps.h"
+
+namespace ps {
+
+// Tree of one model: one root and chains hanging off it
+// (objectdetect_findrot.cpp:207-210 requires every non-root part to have at
+// most one child).
+struct Tree {
+  int P = 0;
+  int root = -1;
+  std::vector<int> parent_joint;  // [P] joint whose child is p, -1 for the root
+  std::vector<int> child_joint;   // [P] joint whose parent is p (non-root), -1 for leaves
+  std::vector<int> root_joints;   // joints with parent == root, in joint order
+  std::vector<int> chain;         // [P] index of the root child whose chain holds p, -1 for root
+  std::vector<int> height;        // [P] 1 for leaves, 1 + height(child) above
+  std::vector<int> depth;         // [P] 0 for the root, 1 for its children, ...
+  int max_height = 0;             // number of upward levels
+  int max_depth = 0;              // number of downward levels
+};
+
+struct Model {
+  int P = 0;
+  std::vector<int> is_detect, is_upright;
+  std::vector<ps_joint> joints;
+  ps_exp_param ep{};
+  int is_sparse = 0;
+  Tree tree;
+};
+
+// Integer plan and taps of one message (computeRotJointMarginal,
+// objectdetect_findrot.cpp:292-456) for a given grid size and scale.
+struct MessagePlan {
+  int delta = 0;                     // rot_mean_idx (:323)
+  int rot_mode = 0;                  // RotMode
+  std::vector<float> rot_taps;       // clipped circular kernel (:377-404)
+  std::vector<float> fx, fy;         // spatial taps, float (multi_array_filter.hpp:246-250)
+  int kx = 0, ky = 0;
+  std::vector<int> fwd_ox, fwd_oy;   // [R] by source slice
+  std::vector<int> inv_ox, inv_oy;   // [R] by slice
+};
+
+// Status + message helpers; every failure path sets the thread's error text.
+int set_error(int status, const std::string &msg);
+
+int build_model(const ps_model_desc *desc, Model &m);
+// joints must be P-1 joints with the model's topology
+int check_joints(const Model &m, const ps_joint *joints);
+int build_message_plan(const ps_exp_param &ep, int H, int W, const double offset_c[2], const double offset_p[2],
+                       const double C[4], double rot_mean, double rot_sigma, double scale, MessagePlan &out);
+
+double rot_from_index(const ps_exp_param &ep, int rotidx);
+double scale_from_index(const ps_exp_param &ep, int scaleidx);
+int index_from_rot(const ps_exp_param &ep, double rot);
+// rotations an upright part keeps: |rot| < 15 degrees (objectdetect_findrot.cpp:509-523)
+unsigned long long upright_masked_bits(const ps_exp_param &ep);
+
+}  // namespace ps

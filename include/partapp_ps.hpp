@@ -1,0 +1,265 @@
+// partapp_ps.hpp — C++ host interface with the reference's signatures.
+//
+// object_detect::computeRootPosteriorRot (objectdetect_findrot.cpp:470-727,
+// objectdetect.h:269-274) and computeRotJointMarginal (:292-456,
+// objectdetect.h:277-282) re-declared as templates over the caller's own
+// types, implemented on the C ABI of partapp_ps.h (HIP device memory is
+// allocated and freed per call, like the reference's per-call grids).
+//
+// The templates read only what the reference reads:
+//   PartAppT   .m_exp_param.{min,max}_part_rotation(), num_rotation_steps(),
+//              {min,max}_object_scale(), num_scale_steps(),
+//              strip_border_detections()
+//              .m_part_conf.part_size(), .part(p).is_detect() / is_upright() / is_root()
+//   JointT     child_idx, parent_idx, offset_c(i), offset_p(i), C(i,j), rot_mean, rot_sigma
+//   Grid3      data(), shape()[0..2] (boost::multi_array<float,3> qualifies)
+//   PartHypT   constructible as PartHypT(exp_param, scaleidx, rotidx, x, y, score)
+// so partapp's PartApp, Joint, FloatGrid3 and PartHyp bind unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "partapp_ps.h"
+
+namespace partapp_ps {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int status, const std::string &where)
+      : std::runtime_error(where + " failed (" + std::to_string(status) + "): " + ps_last_error_string()),
+        status(status) {}
+  int status;
+};
+
+inline void check(int st, const char *where) {
+  if (st != PS_OK) throw Error(st, where);
+}
+inline void check_hip(hipError_t e, const char *where) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// RAII device buffer
+struct DeviceBuffer {
+  void *p = nullptr;
+  explicit DeviceBuffer(size_t bytes) { check_hip(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc"); }
+  ~DeviceBuffer() { (void)hipFree(p); }
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  float *f() const { return static_cast<float *>(p); }
+};
+
+template <class ExpParamT>
+ps_exp_param to_exp_param(const ExpParamT &e) {
+  ps_exp_param p;
+  p.min_part_rotation = e.min_part_rotation();
+  p.max_part_rotation = e.max_part_rotation();
+  p.num_rotation_steps = e.num_rotation_steps();
+  p.min_object_scale = e.min_object_scale();
+  p.max_object_scale = e.max_object_scale();
+  p.num_scale_steps = e.num_scale_steps();
+  p.strip_border_detections = e.strip_border_detections();
+  return p;
+}
+
+template <class JointT>
+ps_joint to_joint(const JointT &j) {
+  ps_joint o;
+  o.child_idx = j.child_idx;
+  o.parent_idx = j.parent_idx;
+  for (int i = 0; i < 2; ++i) {
+    o.offset_c[i] = j.offset_c(i);
+    o.offset_p[i] = j.offset_p(i);
+  }
+  o.C[0] = j.C(0, 0);
+  o.C[1] = j.C(0, 1);
+  o.C[2] = j.C(1, 0);
+  o.C[3] = j.C(1, 1);
+  o.rot_mean = j.rot_mean;
+  o.rot_sigma = j.rot_sigma;
+  return o;
+}
+
+// Owns a ps_model built from the caller's PartApp and joints.
+class Model {
+ public:
+  template <class PartAppT, class JointT>
+  Model(const PartAppT &part_app, int rootpart_idx, const std::vector<JointT> &joints, bool bIsSparse) {
+    const int P = part_app.m_part_conf.part_size();
+    det_.resize(P);
+    up_.resize(P);
+    root_.assign(P, 0);
+    for (int p = 0; p < P; ++p) {
+      det_[p] = part_app.m_part_conf.part(p).is_detect() ? 1 : 0;
+      up_[p] = part_app.m_part_conf.part(p).is_upright() ? 1 : 0;
+    }
+    root_[rootpart_idx] = 1;  // the caller chose the root (objectdetect_findrot.cpp:781-786)
+    for (const JointT &j : joints) joints_.push_back(to_joint(j));
+    ps_model_desc d;
+    d.num_parts = P;
+    d.is_detect = det_.data();
+    d.is_upright = up_.data();
+    d.is_root = root_.data();
+    d.num_joints = (int)joints_.size();
+    d.joints = joints_.data();
+    d.exp_param = to_exp_param(part_app.m_exp_param);
+    d.is_sparse = bIsSparse ? 1 : 0;
+    check(ps_model_create(&d, &m_), "ps_model_create");
+  }
+  ~Model() { ps_model_destroy(m_); }
+  Model(const Model &) = delete;
+  Model &operator=(const Model &) = delete;
+  ps_model *get() const { return m_; }
+
+ private:
+  std::vector<int> det_, up_, root_;
+  std::vector<ps_joint> joints_;
+  ps_model *m_ = nullptr;
+};
+
+// computeRootPosteriorRot with the reference's parameters.  Differences that
+// the reference's callers cannot observe: the upright / border masks are
+// written into log_part_detections exactly as the reference does
+// (objectdetect_findrot.cpp:509-551) but computed on the GPU from an
+// unmasked copy; bSaveMarginals asks for log_part_posterior_out instead of
+// .mat files, and imgidx / flip only name those files in the reference.
+template <class PartAppT, class Grid3, class JointT, class PartHypT>
+void computeRootPosteriorRot(const PartAppT &part_app, std::vector<std::vector<Grid3>> &log_part_detections,
+                             Grid3 &root_part_posterior, int rootpart_idx, std::vector<JointT> joints, bool flip,
+                             bool bIsSparse, int imgidx, std::vector<std::vector<PartHypT>> &best_part_hyp,
+                             bool bSaveMarginals, std::vector<std::vector<Grid3>> *log_part_posterior_out = nullptr) {
+  (void)flip;
+  (void)imgidx;
+  (void)bSaveMarginals;
+  const auto &ep = part_app.m_exp_param;
+  const int P = part_app.m_part_conf.part_size();
+  const int S = (int)ep.num_scale_steps();
+  const int R = (int)ep.num_rotation_steps();
+  if ((int)log_part_detections.size() != P || (int)log_part_detections[0].size() != S)
+    throw std::invalid_argument("log_part_detections must be [nParts][nScales]");
+  const int H = (int)log_part_detections[0][0].shape()[1];
+  const int W = (int)log_part_detections[0][0].shape()[2];
+  const size_t n = (size_t)R * H * W;
+  for (int p = 0; p < P; ++p)
+    for (int s = 0; s < S; ++s) {
+      const auto &g = log_part_detections[p][s];
+      if ((int)g.shape()[0] != R || (int)g.shape()[1] != H || (int)g.shape()[2] != W)
+        throw std::invalid_argument("every score grid must be [nRotations][H][W]");
+    }
+
+  Model model(part_app, rootpart_idx, joints, bIsSparse);
+  std::vector<float> host(n * P * S);
+  for (int p = 0; p < P; ++p)
+    for (int s = 0; s < S; ++s) std::copy(log_part_detections[p][s].data(), log_part_detections[p][s].data() + n,
+                                          host.begin() + ((size_t)p * S + s) * n);
+  DeviceBuffer un(host.size() * sizeof(float)), mg(host.size() * sizeof(float)),
+      rp((size_t)S * H * W * sizeof(float));
+  check_hip(hipMemcpy(un.p, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  ps_instance inst;
+  inst.height = H;
+  inst.width = W;
+  inst.unary_offset = inst.marginal_offset = inst.root_offset = 0;
+  inst.joints = nullptr;
+  std::vector<ps_map> maps((size_t)S * P);
+  check(ps_marginals(model.get(), &inst, 1, un.f(), mg.f(), rp.f(), maps.data(), nullptr), "ps_marginals");
+
+  std::vector<float> root((size_t)S * H * W);
+  check_hip(hipMemcpy(root.data(), rp.p, root.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+  root_part_posterior.resize(std::array<size_t, 3>{(size_t)S, (size_t)H, (size_t)W});
+  std::copy(root.begin(), root.end(), root_part_posterior.data());
+
+  if (log_part_posterior_out) {
+    check_hip(hipMemcpy(host.data(), mg.p, host.size() * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    log_part_posterior_out->assign(P, std::vector<Grid3>(S, log_part_detections[0][0]));
+    for (int p = 0; p < P; ++p)
+      for (int s = 0; s < S; ++s)
+        std::copy(host.begin() + ((size_t)p * S + s) * n, host.begin() + ((size_t)p * S + s + 1) * n,
+                  (*log_part_posterior_out)[p][s].data());
+  }
+
+  // MAP of the last scale (objectdetect_findrot.cpp:257-277 run per scale)
+  best_part_hyp.clear();
+  best_part_hyp.resize(P);
+  for (int p = 0; p < P; ++p) {
+    const ps_map &m = maps[(size_t)(S - 1) * P + p];
+    best_part_hyp[p].push_back(PartHypT(ep, S - 1, m.rotidx, m.x, m.y, m.score));
+  }
+
+  // the reference's in-place masking of the caller's grids (:509-551)
+  for (int p = 0; p < P; ++p) {
+    if (!part_app.m_part_conf.part(p).is_upright()) continue;
+    for (int s = 0; s < S; ++s)
+      for (int r = 0; r < R; ++r) {
+        double step = ((double)ep.max_part_rotation() - (double)ep.min_part_rotation()) / R;
+        double rot = ep.min_part_rotation() == ep.max_part_rotation()
+                         ? (double)ep.min_part_rotation()
+                         : (double)ep.min_part_rotation() + step * (0.5 + r);
+        if (!(std::fabs(rot) < 15.0)) {
+          float *g = log_part_detections[p][s].data() + (size_t)r * H * W;
+          std::fill(g, g + (size_t)H * W, -1e6f);
+        }
+      }
+  }
+  if (ep.strip_border_detections() > 0) {
+    const int strip = (int)(ep.strip_border_detections() * W);
+    for (int s = 0; s < S; ++s)
+      for (size_t row = 0; row < (size_t)R * H; ++row) {
+        float *g = log_part_detections[rootpart_idx][s].data() + row * W;
+        for (int x = 0; x < strip; ++x) g[x] = -1e6f;
+        for (int x = W - strip; x < W; ++x) g[x] = -1e6f;
+      }
+  }
+}
+
+// MAP record of the convenience overload, which discards it.
+struct IgnoredHyp {
+  template <class E>
+  IgnoredHyp(const E &, int, int, int, int, float) {}
+};
+
+// The two-output convenience overload (objectdetect_findrot.cpp:458-468).
+template <class PartAppT, class Grid3, class JointT>
+void computeRootPosteriorRot(const PartAppT &part_app, std::vector<std::vector<Grid3>> &log_part_detections,
+                             Grid3 &root_part_posterior, int rootpart_idx, std::vector<JointT> joints, bool flip,
+                             bool bIsSparse, int imgidx) {
+  std::vector<std::vector<IgnoredHyp>> best;
+  computeRootPosteriorRot(part_app, log_part_detections, root_part_posterior, rootpart_idx, joints, flip, bIsSparse,
+                          imgidx, best, false);
+}
+
+// computeRotJointMarginal: child [R][H][W] -> parent [R][H][W] (host grids).
+template <class ExpParamT, class Grid3, class VecT, class MatT>
+void computeRotJointMarginal(const ExpParamT &exp_param, Grid3 &log_prob_child, Grid3 &log_prob_parent,
+                             const VecT &offset_c_10, const VecT &offset_p_01, const MatT &C, double rot_mean,
+                             double rot_sigma, double scale, bool bIsSparse) {
+  const int R = (int)log_prob_child.shape()[0], H = (int)log_prob_child.shape()[1],
+            W = (int)log_prob_child.shape()[2];
+  const size_t n = (size_t)R * H * W;
+  // a one-joint model carries the exp_param; its tree is irrelevant to ps_message
+  int det[2] = {1, 1}, up[2] = {0, 0}, root[2] = {1, 0};
+  ps_joint j{};
+  j.child_idx = 1;
+  j.parent_idx = 0;
+  j.C[0] = j.C[3] = 1.0;
+  ps_model_desc d{2, det, up, root, 1, &j, to_exp_param(exp_param), bIsSparse ? 1 : 0};
+  ps_model *m = nullptr;
+  check(ps_model_create(&d, &m), "ps_model_create");
+  DeviceBuffer c(n * sizeof(float)), p(n * sizeof(float));
+  double oc[2] = {offset_c_10(0), offset_c_10(1)}, op[2] = {offset_p_01(0), offset_p_01(1)};
+  double cc[4] = {C(0, 0), C(0, 1), C(1, 0), C(1, 1)};
+  int st = PS_OK;
+  hipError_t e = hipMemcpy(c.p, log_prob_child.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) st = ps_message(m, c.f(), p.f(), H, W, oc, op, cc, rot_mean, rot_sigma, scale, nullptr);
+  if (e == hipSuccess && st == PS_OK)
+    e = hipMemcpy(log_prob_parent.data(), p.p, n * sizeof(float), hipMemcpyDeviceToHost);
+  ps_model_destroy(m);
+  check_hip(e, "hipMemcpy");
+  check(st, "ps_message");
+}
+
+}  // namespace partapp_ps

@@ -60,5 +60,27 @@ def build(verbose=False):
     return out
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "wrapper_main.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "wrapper_main")
+
+
+def build_cpp_tests(verbose=False):
+    """The C++ driver of include/partapp_ps.hpp used by the GPU tests."""
+    os.makedirs(os.path.dirname(CPP_TEST_BIN), exist_ok=True)
+    tmp = CPP_TEST_BIN + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           "-o", tmp, CPP_TEST_SRC, "-L", LIBDIR, "-lpartapp_ps", "-Wl,-rpath," + LIBDIR,
+           "-Wl,-rpath,$ORIGIN/../../../partapp_amd/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building %s" % CPP_TEST_BIN)
+    os.replace(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_tests(verbose=True))

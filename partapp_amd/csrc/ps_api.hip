@@ -20,13 +20,14 @@
 namespace ps {
 extern thread_local std::string g_last_error;
 hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map, hipStream_t s);
-hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_rows, hipStream_t s);
+hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
 size_t rowpass_lds_bytes(int R, int W, int kx);
 hipError_t rowpass_set_lds_limit(size_t bytes);
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, size_t lds_bytes, hipStream_t s);
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
+size_t colpass_lds_bytes(int ky);
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, hipStream_t s);
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
 }  // namespace ps
@@ -41,7 +42,8 @@ namespace {
 
 struct Level {
   int off = 0, count = 0, max_h = 0, max_w = 0;
-  size_t lds = 0;
+  size_t lds = 0;      // row pass
+  size_t col_lds = 0;  // column pass
 };
 
 int hip_error(hipError_t e, const char *what) {
@@ -401,8 +403,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         L.max_h = std::max(L.max_h, j.H);
         L.max_w = std::max(L.max_w, j.W);
         L.lds = std::max(L.lds, rowpass_lds_bytes(R, j.W, j.kx));
+        L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
       }
-      if (L.lds > kLdsLimit)
+      if (L.lds > kLdsLimit || L.col_lds > kLdsLimit)
         return set_error(PS_ERR_UNSUPPORTED, "row pass needs " + std::to_string(L.lds) +
                                                  " bytes of LDS (R * width too large for one workgroup)");
       B.jobs.insert(B.jobs.end(), v.begin(), v.end());
@@ -533,16 +536,16 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
   profile_end(pp, ev, s);
   if (e == hipSuccess) {
     ev = profile_begin(pp, 1, s);
-    e = launch_max(b, pl->maxes, pl->n_maxes, pl->R, pl->max_rows, s);
+    e = launch_max(b, pl->maxes, pl->n_maxes, pl->R, pl->max_hw, s);
     profile_end(pp, ev, s);
   }
   auto level = [&](const Level &L) {
     long ev2 = profile_begin(pp, 2, s);
-    e = launch_rowpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.lds, s);
+    e = launch_rowpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
     profile_end(pp, ev2, s);
     if (e != hipSuccess) return;
     ev2 = profile_begin(pp, 3, s);
-    e = launch_colpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, s);
+    e = launch_colpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
     profile_end(pp, ev2, s);
   };
   for (const Level &L : pl->up) {
@@ -721,9 +724,9 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   const int *sd = reinterpret_cast<const int *>(base + o_sh);
   if (e == hipSuccess) e = rowpass_set_lds_limit(lds);
   if (e == hipSuccess) e = launch_reset(b.max_keys, 1, b.map_keys, 1, s);
-  if (e == hipSuccess) e = launch_max(b, reinterpret_cast<const MaxJob *>(base + o_max), 1, R, R * height, s);
-  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, R, height, lds, s);
-  if (e == hipSuccess) e = launch_colpass(b, jd, 1, td, sd, R, height, width, s);
+  if (e == hipSuccess) e = launch_max(b, reinterpret_cast<const MaxJob *>(base + o_max), 1, R, height * width, s);
+  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, R, height, width, lds, s);
+  if (e == hipSuccess) e = launch_colpass(b, jd, 1, td, sd, R, height, width, colpass_lds_bytes(mp.ky), s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(mem);
   if (e != hipSuccess) return hip_error(e, "message");

@@ -21,6 +21,9 @@
 // like a sequential sdot; sums of grids are added in the reference's order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdint>
+
 #include "ps_internal.hpp"
 #include "../../include/partapp_ps.h"
 
@@ -83,25 +86,37 @@ __global__ void k_reset(unsigned *keys, int n_keys, unsigned long long *map_keys
 }
 
 // ------------------------------------------------------------------ k_max
-// Max of a leaf's source grid (getMinMax, multi_array_op.hpp:60-77), one
-// block per group of rows.
+// Max of a leaf's source grid (getMinMax, multi_array_op.hpp:60-77).  Grids
+// without masks are scanned flat with 16-byte loads; masked ones by rows.
 
-__global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__ jobs, int R, int rows_per_block) {
+__global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__ jobs, int R, int chunk) {
   const MaxJob &J = jobs[blockIdx.y];
   const int H = J.H, W = J.W;
-  const int row0 = blockIdx.x * rows_per_block;
+  const long long n = (long long)R * H * W;
+  const long long e0 = (long long)blockIdx.x * chunk;
   unsigned km = 0u;
-  if (J.src_kind == kSrcZero) {
-    km = f2key(0.0f);
-  } else {
-    const float *src = cptr(b, J.src);
-    for (int row = row0; row < min(row0 + rows_per_block, R * H); ++row) {
-      const int r = row / H;
-      const bool rot_masked = (J.mask.rot_bits >> r) & 1ull;
-      const float *p = src + (long long)row * W;
-      for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        float v = (rot_masked || x < J.mask.strip || x >= W - J.mask.strip) ? kLogZero : p[x];
-        km = max(km, f2key(v));
+  if (e0 < n) {
+    const long long e1 = min(e0 + (long long)chunk, n);
+    if (J.src_kind == kSrcZero) {
+      km = f2key(0.0f);
+    } else {
+      const float *src = cptr(b, J.src);
+      const bool plain = J.mask.rot_bits == 0ull && J.mask.strip == 0;
+      if (plain && ((reinterpret_cast<uintptr_t>(src + e0) & 15) == 0) && ((e1 - e0) & 3) == 0) {
+        const float4 *s4 = reinterpret_cast<const float4 *>(src + e0);
+        const int n4 = (int)((e1 - e0) >> 2);
+        for (int i = threadIdx.x; i < n4; i += blockDim.x) {
+          const float4 v = s4[i];
+          km = max(km, max(max(f2key(v.x), f2key(v.y)), max(f2key(v.z), f2key(v.w))));
+        }
+      } else {
+        const long long HW = (long long)H * W;
+        for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+          const int r = (int)(e / HW);
+          const int x = (int)(e % W);
+          const float v = masked(J.mask, r, x, W) ? kLogZero : src[e];
+          km = max(km, f2key(v));
+        }
       }
     }
   }
@@ -110,101 +125,40 @@ __global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__
 }
 
 // -------------------------------------------------------------- k_rowpass
-// LDS: E[R][Wa] (exp of the warped child) and P[R][Pw] (rotation-filtered
-// rows with kx zeros on the left and a zero tail), Pw a multiple of 4.
+// One block per (message, row y), all R slices of the row.  LDS holds
+// P[R][Pw]: the rotation-filtered rows with kx zeros on the left and a zero
+// tail, Pw a multiple of 4, read by the x filter.
 
 struct RowGeom {
-  int Wa, Wq, nchx, Pw;
+  int Wq, nchx, Pw;
 };
 __host__ __device__ __forceinline__ RowGeom row_geom(int W, int kx) {
   RowGeom g;
-  g.Wa = (W + 3) & ~3;
   g.Wq = (W + 3) >> 2;
   g.nchx = (2 * kx + 1 + 3) >> 2;
   g.Pw = 4 * (g.Wq + g.nchx + 1);
   return g;
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_rowpass(Bases b, const MsgJob *__restrict__ jobs,
-                                                const float *__restrict__ taps, const int *__restrict__ shifts,
-                                                int R) {
-  extern __shared__ float4 lds4[];
-  float *lds = reinterpret_cast<float *>(lds4);
-  const MsgJob &J = jobs[blockIdx.y];
-  const int y = blockIdx.x;
-  const int H = J.H, W = J.W;
-  if (y >= H) return;
-  const int kx = J.kx;
-  const RowGeom g = row_geom(W, kx);
-  float *E = lds;
-  float *P = lds + R * g.Wa;
-  const int tid = threadIdx.x;
-
-  const float M = key2f(b.max_keys[J.src_key]);
-  const float negM = -M;
-  const float e_fill = expf(kLogZero + negM);  // exp(LOG_ZERO - M) of the warp's fill value
-  const float *src = cptr(b, J.src);
-  const long long HW = (long long)H * W;
-  const int *fox = shifts + J.shifts;
-  const int *foy = fox + R;
-
-  // zero the halo columns of P
-  for (int i = tid; i < R * (kx + (g.Pw - kx - W)); i += NT) {
-    int r = i / (g.Pw - W);
-    int c = i - r * (g.Pw - W);
-    int p = c < kx ? c : c + W;
-    P[r * g.Pw + p] = 0.0f;
+// zero the halo columns [0, kx) and [kx + W, Pw) of every row of P
+__device__ __forceinline__ void zero_halo(float *P, int R, int W, int kx, const RowGeom &g) {
+  const int per_row = g.Pw - W;
+  for (int i = threadIdx.x; i < R * per_row; i += blockDim.x) {
+    const int r = i / per_row;
+    const int c = i - r * per_row;
+    P[r * g.Pw + (c < kx ? c : c + W)] = 0.0f;
   }
+}
 
-  const int n_rot = J.rot_len >> 1;
-  const float *fr = taps + J.taps_r;
-  for (int x = tid; x < W; x += NT) {
-    // forward warp + exp (objectdetect_findrot.cpp:342-365)
-    for (int ro = 0; ro < R; ++ro) {
-      const int rs = ro - J.delta;
-      float e = e_fill;
-      if (rs >= 0 && rs < R) {
-        const int yy = y + foy[rs];
-        const int xx = x + fox[rs];
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-          float v;
-          if (J.src_kind == kSrcZero) v = 0.0f;
-          else if (J.src_kind == kSrcUnary && masked(J.src_mask, rs, xx, W)) v = kLogZero;
-          else v = src[rs * HW + (long long)yy * W + xx];
-          e = expf(v + negM);
-        }
-      }
-      E[ro * g.Wa + x] = e;
-    }
-    // circular rotation filter (objectdetect_findrot.cpp:377-420,
-    // multi_array_filter.hpp:115-155); the column was written by this thread
-    for (int ro = 0; ro < R; ++ro) {
-      float acc;
-      if (J.rot_mode == kRotFilter) {
-        acc = 0.0f;
-        for (int j = 0; j < J.rot_len; ++j) {
-          int rr = ro - n_rot + j;
-          rr = rr < 0 ? rr + R : (rr >= R ? rr - R : rr);
-          acc = fmaf(E[rr * g.Wa + x], fr[j], acc);
-        }
-      } else if (J.rot_mode == kRotCopy) {
-        acc = E[ro * g.Wa + x];
-      } else {
-        acc = 0.0f;
-      }
-      P[ro * g.Pw + kx + x] = acc;
-    }
-  }
-  __syncthreads();
-
-  // truncated x filter (multi_array_filter.hpp:277-288): four outputs per
-  // lane, taps in chunks of four, a float4 window sliding along the row
-  const float4 *fx4 = reinterpret_cast<const float4 *>(taps + J.taps_x);
-  float *T = mptr(b, J.tmp);
+// truncated x filter (multi_array_filter.hpp:277-288) of every row of P:
+// four outputs per lane, taps in chunks of four (zero padded), a float4
+// window sliding along the row; writes row y of T
+__device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x, float *T, int R, int W,
+                                             long long HW, int y, const RowGeom &g) {
+  const float4 *fx4 = reinterpret_cast<const float4 *>(taps_x);
   const int n_items = R * g.Wq;
   const float inv_wq = 1.0f / (float)g.Wq;
-  for (int item = tid; item < n_items; item += NT) {
+  for (int item = threadIdx.x; item < n_items; item += blockDim.x) {
     int ro = (int)((float)item * inv_wq);
     if (ro * g.Wq > item) --ro;
     if ((ro + 1) * g.Wq <= item) ++ro;
@@ -234,26 +188,212 @@ __global__ __launch_bounds__(NT) void k_rowpass(Bases b, const MsgJob *__restric
   }
 }
 
-// -------------------------------------------------------------- k_colpass
+// Circular rotation filter with LR taps on a register column
+// (objectdetect_findrot.cpp:377-420, multi_array_filter.hpp:115-155):
+// out[r] = sum_j in[(r - n + j) mod R] f[j], taps in ascending order.
+template <int R, int LR>
+__device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], float (&o)[R], const float *fr) {
+  constexpr int n = (LR - 1) / 2;
+  float f[LR];
+#pragma unroll
+  for (int j = 0; j < LR; ++j) f[j] = fr[j];
+#pragma unroll
+  for (int ro = 0; ro < R; ++ro) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < LR; ++j) acc = fmaf(e[((ro - n + j) % R + R) % R], f[j], acc);
+    o[ro] = acc;
+  }
+}
+template <int R, int LR>
+struct RotDispatch {
+  static __device__ __forceinline__ void run(int lr, const float (&e)[R], float (&o)[R], const float *fr) {
+    if (lr == LR) rot_filter_fixed<R, LR>(e, o, fr);
+    else RotDispatch<R, LR - 2>::run(lr, e, o, fr);
+  }
+};
+template <int R>
+struct RotDispatch<R, -1> {
+  static __device__ __forceinline__ void run(int, const float (&)[R], float (&o)[R], const float *) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) o[r] = 0.0f;
+  }
+};
 
-constexpr int kColTileX = 64;   // lanes of a wave: one column each
-constexpr int kColRows = 16;    // outputs per lane
-constexpr int kColGroups = 4;   // waves per block
-constexpr int kColTileY = kColRows * kColGroups;
+// Row pass for a rotation count fixed at compile time: each lane owns one
+// column x; its R source values are loaded together (clamped addresses, then
+// a select), exponentiated, rotation-filtered in registers and written to P.
+template <int R>
+__global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__restrict__ jobs,
+                                                   const float *__restrict__ taps, const int *__restrict__ shifts) {
+  extern __shared__ float4 lds4[];
+  float *P = reinterpret_cast<float *>(lds4);
+  const MsgJob &J = jobs[blockIdx.y];
+  const int y = blockIdx.x;
+  const int H = J.H, W = J.W;
+  if (y >= H) return;
+  const int kx = J.kx;
+  const RowGeom g = row_geom(W, kx);
+  zero_halo(P, R, W, kx, g);
+
+  const float M = key2f(b.max_keys[J.src_key]);
+  const float negM = -M;
+  const float e_fill = expf(kLogZero + negM);  // exp(LOG_ZERO - M): the warp's fill value
+  const float *src = cptr(b, J.src);
+  const long long HW = (long long)H * W;
+  const int *fox = shifts + J.shifts;
+  const int *foy = fox + R;
+  const float *fr = taps + J.taps_r;
+  const int delta = J.delta, kind = J.src_kind;
+
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    // forward warp (objectdetect_findrot.cpp:342-359): output slice ro reads
+    // source slice ro - delta at (y + foy, x + fox), else LOG_ZERO
+    float v[R];
+    bool ok[R];
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = ro - delta;
+      const bool rs_ok = rs >= 0 && rs < R;
+      const int rsc = rs_ok ? rs : 0;
+      const int yy = y + foy[rsc];
+      const int xx = x + fox[rsc];
+      ok[ro] = rs_ok && yy >= 0 && yy < H && xx >= 0 && xx < W;
+      const long long off = ok[ro] ? rsc * HW + (long long)yy * W + xx : 0;
+      v[ro] = src[off];
+      if (kind == kSrcUnary && masked(J.src_mask, rsc, xx, W)) v[ro] = kLogZero;
+      if (kind == kSrcZero) v[ro] = 0.0f;
+    }
+    // exp(b - M) (:362-365)
+    float e[R];
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) e[ro] = ok[ro] ? expf(v[ro] + negM) : e_fill;
+    float o[R];
+    if (J.rot_mode == kRotFilter) {
+      RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1>::run(J.rot_len, e, o, fr);
+    } else if (J.rot_mode == kRotCopy) {
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) o[ro] = e[ro];
+    } else {
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) o[ro] = 0.0f;
+    }
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) P[ro * g.Pw + kx + x] = o[ro];
+  }
+  __syncthreads();
+  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, W, HW, y, g);
+}
+
+// Row pass for any rotation count: the exponentiated column goes through an
+// LDS buffer E[R][Wa] before the rotation filter.
+__global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__restrict__ jobs,
+                                                     const float *__restrict__ taps, const int *__restrict__ shifts,
+                                                     int R) {
+  extern __shared__ float4 lds4[];
+  const MsgJob &J = jobs[blockIdx.y];
+  const int y = blockIdx.x;
+  const int H = J.H, W = J.W;
+  if (y >= H) return;
+  const int kx = J.kx;
+  const RowGeom g = row_geom(W, kx);
+  const int Wa = 4 * g.Wq;
+  float *E = reinterpret_cast<float *>(lds4);
+  float *P = E + R * Wa;
+  zero_halo(P, R, W, kx, g);
+
+  const float M = key2f(b.max_keys[J.src_key]);
+  const float negM = -M;
+  const float e_fill = expf(kLogZero + negM);
+  const float *src = cptr(b, J.src);
+  const long long HW = (long long)H * W;
+  const int *fox = shifts + J.shifts;
+  const int *foy = fox + R;
+  const int n_rot = J.rot_len >> 1;
+  const float *fr = taps + J.taps_r;
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = ro - J.delta;
+      float e = e_fill;
+      if (rs >= 0 && rs < R) {
+        const int yy = y + foy[rs];
+        const int xx = x + fox[rs];
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+          float v;
+          if (J.src_kind == kSrcZero) v = 0.0f;
+          else if (J.src_kind == kSrcUnary && masked(J.src_mask, rs, xx, W)) v = kLogZero;
+          else v = src[rs * HW + (long long)yy * W + xx];
+          e = expf(v + negM);
+        }
+      }
+      E[ro * Wa + x] = e;
+    }
+    // the column was written by this thread: no barrier needed
+    for (int ro = 0; ro < R; ++ro) {
+      float acc;
+      if (J.rot_mode == kRotFilter) {
+        acc = 0.0f;
+        for (int j = 0; j < J.rot_len; ++j) {
+          int rr = ro - n_rot + j;
+          rr = rr < 0 ? rr + R : (rr >= R ? rr - R : rr);
+          acc = fmaf(E[rr * Wa + x], fr[j], acc);
+        }
+      } else if (J.rot_mode == kRotCopy) {
+        acc = E[ro * Wa + x];
+      } else {
+        acc = 0.0f;
+      }
+      P[ro * g.Pw + kx + x] = acc;
+    }
+  }
+  __syncthreads();
+  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, W, HW, y, g);
+}
+
+// -------------------------------------------------------------- k_colpass
+// One block per (message, slice r, strip of 64 columns); it walks down the
+// strip in tiles of 64 output rows.  Each tile stages the source rows it needs
+// (64 + padded taps, zero outside the grid, inverse shift folded into the
+// origin) in LDS; each of the four waves then computes 16 rows, one column per
+// lane, with a 16-row register ring fed from LDS.  Taps run in chunks of 8
+// (the tap pool is zero padded), so the ring indices are static.
+
+constexpr int kColTileX = 64;
+constexpr int kColRows = 16;
+constexpr int kColWaves = 4;
+constexpr int kColTileY = kColRows * kColWaves;
+
+__host__ __device__ __forceinline__ int col_padded_taps(int ky) { return (2 * ky + 1 + 7) & ~7; }
+__host__ __device__ __forceinline__ int col_tile_rows(int ky) { return kColTileY + col_padded_taps(ky); }
+
+// 8 taps of the y filter starting at tap t; PH is the ring phase (t % 16)
+template <int PH>
+__device__ __forceinline__ void col_chunk8(float (&acc)[kColRows], float (&buf)[kColRows], const float *fy,
+                                           const float *tw, int t) {
+  float f[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) f[u] = fy[t + u];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+#pragma unroll
+    for (int i = 0; i < kColRows; ++i) acc[i] = fmaf(buf[(i + u + PH) & (kColRows - 1)], f[u], acc[i]);
+    buf[(u + PH) & (kColRows - 1)] = tw[(t + u + kColRows) * kColTileX];
+  }
+}
 
 __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restrict__ jobs,
                                                  const float *__restrict__ taps, const int *__restrict__ shifts,
-                                                 int R, int ytiles) {
-  const MsgJob &J = jobs[blockIdx.z];
+                                                 int R, int xtiles) {
+  extern __shared__ float4 lds4[];
+  float *tile = reinterpret_cast<float *>(lds4);
+  const MsgJob &J = jobs[blockIdx.y];
   const int H = J.H, W = J.W;
-  const int r = blockIdx.y / ytiles;
-  const int yt = blockIdx.y - r * ytiles;
+  const int r = blockIdx.x / xtiles;
+  const int xt = blockIdx.x - r * xtiles;
+  if (xt * kColTileX >= W) return;
   const int lane = threadIdx.x & 63;
-  const int x = blockIdx.x * kColTileX + lane;
-  const int y0 = yt * kColTileY + (threadIdx.x >> 6) * kColRows;
-  // whole waves outside the grid leave; the others keep every lane for the
-  // wave reductions
-  if (blockIdx.x * kColTileX >= W || y0 >= H) return;
+  const int wave = threadIdx.x >> 6;
+  const int x = xt * kColTileX + lane;
   const bool xin = x < W;
 
   const int *sh = shifts + J.shifts;
@@ -262,73 +402,76 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   const int xs = x + ox;
   const bool col_ok = xin && xs >= 0 && xs < W;
   const long long HW = (long long)H * W;
-  const float *T = cptr(b, J.tmp) + r * HW + (col_ok ? xs : 0);
-  const int ky = J.ky, L = 2 * ky + 1;
-  const int q0 = y0 + oy - ky;  // source row of tap 0 for output 0
+  const float *Tr = cptr(b, J.tmp) + r * HW;
+  const int ky = J.ky;
+  const int Lp = col_padded_taps(ky);
+  const int nrows = col_tile_rows(ky);
   const float *fy = taps + J.taps_y;
-
-  auto load = [&](int q) -> float {
-    const int row = q0 + q;
-    return (col_ok && row >= 0 && row < H) ? T[(long long)row * W] : 0.0f;
-  };
-
-  float buf[kColRows], acc[kColRows];
-#pragma unroll
-  for (int i = 0; i < kColRows; ++i) {
-    buf[i] = load(i);
-    acc[i] = 0.0f;
-  }
-  // tap t of output i reads source row i + t, kept in slot (i + t) % 16
-  for (int t0 = 0; t0 < L; t0 += kColRows) {
-#pragma unroll
-    for (int u = 0; u < kColRows; ++u) {
-      const int t = t0 + u;
-      if (t < L) {
-        const float f = fy[t];
-#pragma unroll
-        for (int i = 0; i < kColRows; ++i) acc[i] = fmaf(buf[(i + u) & (kColRows - 1)], f, acc[i]);
-        buf[u] = load(t + kColRows);
-      }
-    }
-  }
-
   const float M = key2f(b.max_keys[J.src_key]);
-  unsigned kmax1 = 0u, kmax2 = 0u;
-  unsigned long long amax = 0ull;
   float *dst = mptr(b, J.dst);
   const float *dun = J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : nullptr;
   float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
-  const bool rmask = (J.dst_mask.rot_bits >> r) & 1ull;
-  const bool xmask = rmask || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
+  const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
+  // column loaded by this thread's lane slot, and whether it exists
+  const int lcol = xt * kColTileX + (threadIdx.x & 63) + ox;
+  const bool lcol_ok = (xt * kColTileX + (threadIdx.x & 63)) < W && lcol >= 0 && lcol < W;
 
+  unsigned kmax1 = 0u, kmax2 = 0u;
+  unsigned long long amax = 0ull;
+
+  for (int ty0 = 0; ty0 < H; ty0 += kColTileY) {
+    const int s0 = ty0 + oy - ky;  // source row of tap 0 for output row ty0
+    __syncthreads();
+    for (int rr = threadIdx.x >> 6; rr < nrows; rr += kColWaves) {
+      const int row = s0 + rr;
+      tile[rr * kColTileX + (threadIdx.x & 63)] =
+          (lcol_ok && row >= 0 && row < H) ? Tr[(long long)row * W + lcol] : 0.0f;
+    }
+    __syncthreads();
+
+    const float *tw = tile + wave * kColRows * kColTileX + lane;
+    float acc[kColRows], buf[kColRows];
 #pragma unroll
-  for (int i = 0; i < kColRows; ++i) {
-    const int y = y0 + i;
-    if (!xin || y >= H) continue;
-    const int ysrc = y + oy;
-    // inverse warp (objectdetect_findrot.cpp:437-448): outside -> LOG_ZERO
-    const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(acc[i]) + M : kLogZero;
-    const long long e = r * HW + (long long)y * W + x;
-    if (J.dst_kind == kDstUpRoot) {
-      dst[e] = msg;
-    } else if (J.dst_kind == kDstUpNode) {
-      // belief of a chain node: 0 + message (+ unary), :637, :652-654
-      float v = msg;
-      if (J.dst_add_unary) v = v + (xmask ? kLogZero : dun[e]);
-      dst[e] = v;
-      kmax1 = max(kmax1, f2key(v));
-    } else {
-      // marginal = belief + downward message (:201); next source = unary + message (:220-222)
-      const float u = xmask ? kLogZero : dun[e];
-      const float base = J.dst_base == 0 ? dst[e] : (J.dst_base == 1 ? u : 0.0f);
-      const float mg = base + msg;
-      dst[e] = mg;
-      const unsigned long long k = amax_key(mg, (unsigned)e);
-      amax = k > amax ? k : amax;
-      if (dst2) {
-        const float v2 = u + msg;
-        dst2[e] = v2;
-        kmax2 = max(kmax2, f2key(v2));
+    for (int i = 0; i < kColRows; ++i) {
+      acc[i] = 0.0f;
+      buf[i] = tw[i * kColTileX];
+    }
+    for (int t0 = 0; t0 < Lp; t0 += 2 * 8) {
+      col_chunk8<0>(acc, buf, fy, tw, t0);
+      if (t0 + 8 >= Lp) break;
+      col_chunk8<8>(acc, buf, fy, tw, t0 + 8);
+    }
+
+    const int y0 = ty0 + wave * kColRows;
+#pragma unroll
+    for (int i = 0; i < kColRows; ++i) {
+      const int y = y0 + i;
+      if (!xin || y >= H) continue;
+      const int ysrc = y + oy;
+      // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
+      const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(acc[i]) + M : kLogZero;
+      const long long e = r * HW + (long long)y * W + x;
+      if (J.dst_kind == kDstUpRoot) {
+        dst[e] = msg;
+      } else if (J.dst_kind == kDstUpNode) {
+        // belief of a chain node: 0 + message (+ unary), :637, :652-654
+        float v = msg;
+        if (J.dst_add_unary) v = v + (xmask ? kLogZero : dun[e]);
+        dst[e] = v;
+        kmax1 = max(kmax1, f2key(v));
+      } else {
+        // marginal = belief + downward message (:201); next source = unary + message (:220-222)
+        const float u = xmask ? kLogZero : dun[e];
+        const float base = J.dst_base == 0 ? dst[e] : (J.dst_base == 1 ? u : 0.0f);
+        const float mg = base + msg;
+        dst[e] = mg;
+        const unsigned long long k = amax_key(mg, (unsigned)e);
+        amax = k > amax ? k : amax;
+        if (dst2) {
+          const float v2 = u + msg;
+          dst2[e] = v2;
+          kmax2 = max(kmax2, f2key(v2));
+        }
       }
     }
   }
@@ -452,39 +595,54 @@ hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys
   return hipGetLastError();
 }
 
-hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_rows, hipStream_t s) {
+hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
-  const int rows_per_block = 8;
-  dim3 grid((max_rows + rows_per_block - 1) / rows_per_block, n_jobs);
-  hipLaunchKernelGGL(k_max, grid, dim3(256), 0, s, b, jobs, R, rows_per_block);
+  const int chunk = 16384;  // elements per block, a multiple of 4
+  const long long n = (long long)R * max_hw;
+  dim3 grid((unsigned)((n + chunk - 1) / chunk), n_jobs);
+  hipLaunchKernelGGL(k_max, grid, dim3(256), 0, s, b, jobs, R, chunk);
   return hipGetLastError();
 }
 
+static bool rowpass_fixed(int R) { return R == 24; }
+
 size_t rowpass_lds_bytes(int R, int W, int kx) {
   RowGeom g = row_geom(W, kx);
-  return sizeof(float) * ((size_t)R * g.Wa + (size_t)R * g.Pw);
+  size_t p = (size_t)R * g.Pw;
+  if (!rowpass_fixed(R)) p += (size_t)R * 4 * g.Wq;  // the E buffer of the generic kernel
+  return sizeof(float) * p;
 }
 
 hipError_t rowpass_set_lds_limit(size_t bytes) {
   if (bytes <= 64 * 1024) return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rowpass<256>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rowpass_r<24>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rowpass_any),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, size_t lds_bytes, hipStream_t s) {
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
   dim3 grid(max_h, n_jobs);
-  hipLaunchKernelGGL(k_rowpass<256>, grid, dim3(256), lds_bytes, s, b, jobs, taps, shifts, R);
+  const int nt = std::min(512, (max_w + 63) / 64 * 64);
+  if (R == 24)
+    hipLaunchKernelGGL(k_rowpass_r<24>, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
+  else
+    hipLaunchKernelGGL(k_rowpass_any, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts, R);
   return hipGetLastError();
 }
 
+size_t colpass_lds_bytes(int ky) { return sizeof(float) * (size_t)col_tile_rows(ky) * kColTileX; }
+
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, hipStream_t s) {
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s) {
+  (void)max_h;
   if (n_jobs <= 0) return hipSuccess;
-  const int ytiles = (max_h + kColTileY - 1) / kColTileY;
-  dim3 grid((max_w + kColTileX - 1) / kColTileX, R * ytiles, n_jobs);
-  hipLaunchKernelGGL(k_colpass, grid, dim3(256), 0, s, b, jobs, taps, shifts, R, ytiles);
+  const int xtiles = (max_w + kColTileX - 1) / kColTileX;
+  dim3 grid(R * xtiles, n_jobs);
+  hipLaunchKernelGGL(k_colpass, grid, dim3(kColWaves * 64), lds_bytes, s, b, jobs, taps, shifts, R, xtiles);
   return hipGetLastError();
 }
 

@@ -147,6 +147,26 @@ int ps_message(const ps_model *model, const float *child_dev, float *parent_dev,
                const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
                double rot_sigma, double scale, void *stream);
 
+/* findLocalMax (objectdetect_aux.cpp:193-251) on device grids.  Grid g is
+ * [D][H][W] floats at grids_dev + offsets[g] (dims[3g..3g+2] = D, H, W; D is
+ * rotations for a part marginal, scales for the root posterior).  A point is
+ * kept when no spatial 8-neighbour in its slice is strictly greater and the
+ * neighbouring slices are strictly lower there.  At most max_n points per
+ * grid: in scan order (d, then x, then y) when there are at most max_n,
+ * otherwise the max_n largest by value, in decreasing order (ties in scan
+ * order).  out_dev: [n_grids][max_n] records, count_dev: [n_grids].
+ * max_n <= 4096.  Allocates a workspace of 8 bytes per grid point; synchronous.
+ * Feeds best_part_hyp (objectdetect_findrot.cpp:280-283) and the root
+ * HypothesisList (:1037-1048). */
+typedef struct ps_local_max {
+  int d;  /* slice: rotation index (part marginal) or scale index (root posterior) */
+  int x;
+  int y;
+  float score;
+} ps_local_max;
+int ps_find_local_max(const float *grids_dev, const long long *offsets, const int *dims, int n_grids, int max_n,
+                      ps_local_max *out_dev, int *count_dev, void *stream);
+
 /* --------------------------------------------------------------- utilities */
 const char *ps_last_error_string(void);
 const char *ps_version(void);

@@ -9,7 +9,7 @@
 // The templates read only what the reference reads:
 //   PartAppT   .m_exp_param.{min,max}_part_rotation(), num_rotation_steps(),
 //              {min,max}_object_scale(), num_scale_steps(),
-//              strip_border_detections()
+//              strip_border_detections(), roi_save_num_samples()
 //              .m_part_conf.part_size(), .part(p).is_detect() / is_upright() / is_root()
 //   JointT     child_idx, parent_idx, offset_c(i), offset_p(i), C(i,j), rot_mean, rot_sigma
 //   Grid3      data(), shape()[0..2] (boost::multi_array<float,3> qualifies)
@@ -182,12 +182,38 @@ void computeRootPosteriorRot(const PartAppT &part_app, std::vector<std::vector<G
                   (*log_part_posterior_out)[p][s].data());
   }
 
-  // MAP of the last scale (objectdetect_findrot.cpp:257-277 run per scale)
+  // MAP of the last scale, then its local maxima (objectdetect_findrot.cpp:257-283
+  // run per scale; findLocalMax tags its hypotheses with scale index 0,
+  // objectdetect_aux.cpp:304)
   best_part_hyp.clear();
   best_part_hyp.resize(P);
+  const int max_n = (int)ep.roi_save_num_samples();
+  std::vector<ps_local_max> lm;
+  std::vector<int> lm_n(P, 0);
+  if (max_n > 0) {
+    std::vector<long long> offs(P);
+    std::vector<int> dims(3 * P);
+    for (int p = 0; p < P; ++p) {
+      offs[p] = ((long long)p * S + (S - 1)) * (long long)n;
+      dims[3 * p] = R;
+      dims[3 * p + 1] = H;
+      dims[3 * p + 2] = W;
+    }
+    DeviceBuffer lmd(sizeof(ps_local_max) * (size_t)P * max_n), cnd(sizeof(int) * P);
+    check(ps_find_local_max(mg.f(), offs.data(), dims.data(), P, max_n, static_cast<ps_local_max *>(lmd.p),
+                            static_cast<int *>(cnd.p), nullptr),
+          "ps_find_local_max");
+    lm.resize((size_t)P * max_n);
+    check_hip(hipMemcpy(lm.data(), lmd.p, lm.size() * sizeof(ps_local_max), hipMemcpyDeviceToHost), "hipMemcpy");
+    check_hip(hipMemcpy(lm_n.data(), cnd.p, sizeof(int) * P, hipMemcpyDeviceToHost), "hipMemcpy");
+  }
   for (int p = 0; p < P; ++p) {
     const ps_map &m = maps[(size_t)(S - 1) * P + p];
     best_part_hyp[p].push_back(PartHypT(ep, S - 1, m.rotidx, m.x, m.y, m.score));
+    for (int i = 0; i < lm_n[p]; ++i) {
+      const ps_local_max &h = lm[(size_t)p * max_n + i];
+      best_part_hyp[p].push_back(PartHypT(ep, 0, h.d, h.x, h.y, h.score));
+    }
   }
 
   // the reference's in-place masking of the caller's grids (:509-551)

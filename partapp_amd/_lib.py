@@ -85,6 +85,15 @@ class ps_map(ctypes.Structure):
     ]
 
 
+class ps_local_max(ctypes.Structure):
+    _fields_ = [
+        ("d", ctypes.c_int),
+        ("x", ctypes.c_int),
+        ("y", ctypes.c_int),
+        ("score", ctypes.c_float),
+    ]
+
+
 # name -> (restype, argtypes); the exported surface of include/partapp_ps.h
 SIGNATURES = {
     "ps_model_create": (ctypes.c_int, [ctypes.POINTER(ps_model_desc), ctypes.POINTER(ctypes.c_void_p)]),
@@ -104,6 +113,8 @@ SIGNATURES = {
     "ps_message": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   c_double_p, c_double_p, c_double_p, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_void_p]),
+    "ps_find_local_max": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), c_int_p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_last_error_string": (ctypes.c_char_p, []),
     "ps_version": (ctypes.c_char_p, []),
     "ps_debug_message_plan": (ctypes.c_int, [ctypes.POINTER(ps_exp_param), ctypes.c_int, ctypes.c_int, c_double_p,

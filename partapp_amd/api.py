@@ -296,6 +296,36 @@ def compute_rot_joint_marginal(model: PartModel, log_prob_child, offset_c, offse
     return parent
 
 
+LOCAL_MAX_DTYPE = np.dtype([("d", np.int32), ("x", np.int32), ("y", np.int32), ("score", np.float32)])
+
+
+def find_local_max(grids, max_n=1000, stream=None):
+    """findLocalMax (objectdetect_aux.cpp:193-251) of device grids.
+
+    grids: a torch CUDA float32 tensor [D][H][W] or a list of them (each
+    contiguous).  Returns, per grid, a structured array of (d, x, y, score):
+    all local maxima in scan order when there are at most max_n, else the
+    max_n largest in decreasing order."""
+    import torch
+    single = hasattr(grids, "shape") and len(grids.shape) == 3
+    gl = [grids] if single else list(grids)
+    base = gl[0].data_ptr()
+    dev = gl[0].device
+    # one address space: offsets relative to the lowest-addressed grid
+    lo = min(g.data_ptr() for g in gl)
+    offs = (ctypes.c_longlong * len(gl))(*[(g.data_ptr() - lo) // 4 for g in gl])
+    dims = (ctypes.c_int * (3 * len(gl)))(*[v for g in gl for v in g.shape])
+    out = torch.empty((len(gl) * max_n * 4,), dtype=torch.int32, device=dev)
+    cnt = torch.empty((len(gl),), dtype=torch.int32, device=dev)
+    check(lib().ps_find_local_max(lo, offs, dims, len(gl), int(max_n), out.data_ptr(), cnt.data_ptr(),
+                                  _stream_handle(stream)), "ps_find_local_max")
+    del base
+    rec = np.frombuffer(out.cpu().numpy().tobytes(), dtype=LOCAL_MAX_DTYPE).reshape(len(gl), max_n)
+    n = cnt.cpu().numpy()
+    res = [rec[i, :n[i]].copy() for i in range(len(gl))]
+    return res[0] if single else res
+
+
 # ----------------------------------------------------------- synthetic data
 
 PERSON_ROOT = 4

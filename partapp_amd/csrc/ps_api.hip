@@ -182,6 +182,17 @@ struct Builder {
     shifts.insert(shifts.end(), mp.fwd_oy.begin(), mp.fwd_oy.end());
     shifts.insert(shifts.end(), mp.inv_ox.begin(), mp.inv_ox.end());
     shifts.insert(shifts.end(), mp.inv_oy.begin(), mp.inv_oy.end());
+    // the forward offsets again, indexed by output slice (source = output - delta);
+    // an output slice without a source gets a row offset no grid can satisfy
+    const int R = (int)mp.fwd_ox.size();
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = ro - mp.delta;
+      shifts.push_back(rs >= 0 && rs < R ? mp.fwd_ox[rs] : 0);
+    }
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = ro - mp.delta;
+      shifts.push_back(rs >= 0 && rs < R ? mp.fwd_oy[rs] : kNoSourceRow);
+    }
     return PS_OK;
   }
 };

@@ -11,6 +11,7 @@ constexpr float kLogZero = -1e6f;  // LOG_ZERO, boost_math.h:23
 constexpr int kMaxRot = 64;        // rotation masks are 64-bit words
 constexpr int kMaxRootChildren = 16;
 constexpr int kMaxTaps = 1000;     // multi_array_filter.hpp:243-244
+constexpr int kNoSourceRow = -(1 << 30);  // row offset of an output slice with no source slice
 
 // Which base pointer an offset is relative to.
 enum BufId : int { kBufUnary = 0, kBufMarginal = 1, kBufScratch = 2, kBufRoot = 3 };
@@ -55,7 +56,8 @@ struct MsgJob {
   int taps_r;    // offsets into the tap pool (floats); x/y tap runs are padded
   int taps_x;    //   with zeros to a multiple of 16
   int taps_y;
-  int shifts;    // offset into the shift pool: fwd_ox[R] fwd_oy[R] inv_ox[R] inv_oy[R]
+  int shifts;    // offset into the shift pool: fwd_ox[R] fwd_oy[R] inv_ox[R] inv_oy[R] (by source
+                 // slice), then fwd_ox[R] fwd_oy[R] by output slice
   int src_kind;  // SrcKind
   int src_key;   // index of the source's max key
   GridRef src;   // [R][H][W]

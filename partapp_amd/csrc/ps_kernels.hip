@@ -130,13 +130,15 @@ __global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__
 // tail, Pw a multiple of 4, read by the x filter.
 
 struct RowGeom {
-  int Wq, nchx, Pw;
+  int Wo;    // groups of 8 outputs per row
+  int nch;   // chunks of 8 taps
+  int Pw;    // padded row pitch of P, a multiple of 8
 };
 __host__ __device__ __forceinline__ RowGeom row_geom(int W, int kx) {
   RowGeom g;
-  g.Wq = (W + 3) >> 2;
-  g.nchx = (2 * kx + 1 + 3) >> 2;
-  g.Pw = 4 * (g.Wq + g.nchx + 1);
+  g.Wo = (W + 7) >> 3;
+  g.nch = (2 * kx + 1 + 7) >> 3;
+  g.Pw = 8 * (g.Wo + g.nch + 1);
   return g;
 }
 
@@ -151,39 +153,46 @@ __device__ __forceinline__ void zero_halo(float *P, int R, int W, int kx, const 
 }
 
 // truncated x filter (multi_array_filter.hpp:277-288) of every row of P:
-// four outputs per lane, taps in chunks of four (zero padded), a float4
-// window sliding along the row; writes row y of T
+// eight outputs per lane, taps in chunks of eight (zero padded), a 16-float
+// window sliding along the row in two 16-byte LDS reads per chunk; writes
+// row y of T
 __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x, float *T, int R, int W,
                                              long long HW, int y, const RowGeom &g) {
-  const float4 *fx4 = reinterpret_cast<const float4 *>(taps_x);
-  const int n_items = R * g.Wq;
-  const float inv_wq = 1.0f / (float)g.Wq;
+  const int n_items = R * g.Wo;
+  const float inv_wo = 1.0f / (float)g.Wo;
   for (int item = threadIdx.x; item < n_items; item += blockDim.x) {
-    int ro = (int)((float)item * inv_wq);
-    if (ro * g.Wq > item) --ro;
-    if ((ro + 1) * g.Wq <= item) ++ro;
-    const int q = item - ro * g.Wq;
-    const float4 *P4 = reinterpret_cast<const float4 *>(P + ro * g.Pw);
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    float4 lo = P4[q];
-    for (int c = 0; c < g.nchx; ++c) {
-      const float4 hi = P4[q + c + 1];
-      const float4 f = fx4[c];
-      a0 = fmaf(lo.x, f.x, a0); a1 = fmaf(lo.y, f.x, a1); a2 = fmaf(lo.z, f.x, a2); a3 = fmaf(lo.w, f.x, a3);
-      a0 = fmaf(lo.y, f.y, a0); a1 = fmaf(lo.z, f.y, a1); a2 = fmaf(lo.w, f.y, a2); a3 = fmaf(hi.x, f.y, a3);
-      a0 = fmaf(lo.z, f.z, a0); a1 = fmaf(lo.w, f.z, a1); a2 = fmaf(hi.x, f.z, a2); a3 = fmaf(hi.y, f.z, a3);
-      a0 = fmaf(lo.w, f.w, a0); a1 = fmaf(hi.x, f.w, a1); a2 = fmaf(hi.y, f.w, a2); a3 = fmaf(hi.z, f.w, a3);
-      lo = hi;
+    int ro = (int)((float)item * inv_wo);
+    if (ro * g.Wo > item) --ro;
+    if ((ro + 1) * g.Wo <= item) ++ro;
+    const int q = item - ro * g.Wo;
+    const float4 *P4 = reinterpret_cast<const float4 *>(P + ro * g.Pw) + 2 * q;
+    float a[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = 0.0f;
+    float4 w0 = P4[0], w1 = P4[1];
+    for (int c = 0; c < g.nch; ++c) {
+      const float4 w2 = P4[2 * c + 2], w3 = P4[2 * c + 3];
+      const float win[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                             w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+      float f[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f[u] = taps_x[8 * c + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = fmaf(win[i + u], f[u], a[i]);
+      w0 = w2;
+      w1 = w3;
     }
     float *trow = T + ro * HW + (long long)y * W;
-    const int x0 = 4 * q;
-    if ((W & 3) == 0) {
-      *reinterpret_cast<float4 *>(trow + x0) = make_float4(a0, a1, a2, a3);
+    const int x0 = 8 * q;
+    if ((W & 7) == 0) {
+      *reinterpret_cast<float4 *>(trow + x0) = make_float4(a[0], a[1], a[2], a[3]);
+      *reinterpret_cast<float4 *>(trow + x0 + 4) = make_float4(a[4], a[5], a[6], a[7]);
     } else {
-      if (x0 + 0 < W) trow[x0 + 0] = a0;
-      if (x0 + 1 < W) trow[x0 + 1] = a1;
-      if (x0 + 2 < W) trow[x0 + 2] = a2;
-      if (x0 + 3 < W) trow[x0 + 3] = a3;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (x0 + i < W) trow[x0 + i] = a[i];
     }
   }
 }
@@ -241,33 +250,44 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
   const float e_fill = expf(kLogZero + negM);  // exp(LOG_ZERO - M): the warp's fill value
   const float *src = cptr(b, J.src);
   const long long HW = (long long)H * W;
-  const int *fox = shifts + J.shifts;
-  const int *foy = fox + R;
+  // per OUTPUT slice: the source slice is ro - delta, read at (y + oyo, x + oxo);
+  // slices without a source carry an offset that fails the bounds test
+  const int *oxo = shifts + J.shifts + 4 * R;
+  const int *oyo = oxo + R;
   const float *fr = taps + J.taps_r;
-  const int delta = J.delta, kind = J.src_kind;
+  const int delta = J.delta;
+  const bool zero_src = J.src_kind == kSrcZero;
+  const UnaryMask mk = J.src_mask;
+  const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
+  const int HW32 = H * W;  // R * H * W < 2^31 is checked when the plan is built
 
   for (int x = threadIdx.x; x < W; x += blockDim.x) {
-    // forward warp (objectdetect_findrot.cpp:342-359): output slice ro reads
-    // source slice ro - delta at (y + foy, x + fox), else LOG_ZERO
-    float v[R];
-    bool ok[R];
+    // forward warp (objectdetect_findrot.cpp:342-359); the loads of all
+    // slices are issued before any of them is used.  off < 0: no source
+    // pixel, the warp's LOG_ZERO fill.
+    int off[R];
 #pragma unroll
     for (int ro = 0; ro < R; ++ro) {
-      const int rs = ro - delta;
-      const bool rs_ok = rs >= 0 && rs < R;
-      const int rsc = rs_ok ? rs : 0;
-      const int yy = y + foy[rsc];
-      const int xx = x + fox[rsc];
-      ok[ro] = rs_ok && yy >= 0 && yy < H && xx >= 0 && xx < W;
-      const long long off = ok[ro] ? rsc * HW + (long long)yy * W + xx : 0;
-      v[ro] = src[off];
-      if (kind == kSrcUnary && masked(J.src_mask, rsc, xx, W)) v[ro] = kLogZero;
-      if (kind == kSrcZero) v[ro] = 0.0f;
+      const int yy = y + oyo[ro];
+      const int xx = x + oxo[ro];
+      const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+      off[ro] = ok ? (ro - delta) * HW32 + yy * W + xx : -1;
     }
-    // exp(b - M) (:362-365)
     float e[R];
+    if (zero_src) {
 #pragma unroll
-    for (int ro = 0; ro < R; ++ro) e[ro] = ok[ro] ? expf(v[ro] + negM) : e_fill;
+      for (int ro = 0; ro < R; ++ro) e[ro] = 0.0f;
+    } else {
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) e[ro] = src[off[ro] < 0 ? 0 : off[ro]];
+    }
+    // masks, then exp(b - M) (:362-365)
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) {
+      float vv = e[ro];
+      if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
+      e[ro] = off[ro] < 0 ? e_fill : expf(vv + negM);
+    }
     float o[R];
     if (J.rot_mode == kRotFilter) {
       RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1>::run(J.rot_len, e, o, fr);
@@ -297,7 +317,7 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
   if (y >= H) return;
   const int kx = J.kx;
   const RowGeom g = row_geom(W, kx);
-  const int Wa = 4 * g.Wq;
+  const int Wa = (W + 3) & ~3;
   float *E = reinterpret_cast<float *>(lds4);
   float *P = E + R * Wa;
   zero_halo(P, R, W, kx, g);
@@ -365,6 +385,8 @@ constexpr int kColTileY = kColRows * kColWaves;
 
 __host__ __device__ __forceinline__ int col_padded_taps(int ky) { return (2 * ky + 1 + 7) & ~7; }
 __host__ __device__ __forceinline__ int col_tile_rows(int ky) { return kColTileY + col_padded_taps(ky); }
+// rows moved per thread and pass when a tile hands its halo to the next one
+constexpr int kColMove = 24;
 
 // 8 taps of the y filter starting at tap t; PH is the ring phase (t % 16)
 template <int PH>
@@ -413,21 +435,34 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
   const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
   // column loaded by this thread's lane slot, and whether it exists
-  const int lcol = xt * kColTileX + (threadIdx.x & 63) + ox;
-  const bool lcol_ok = (xt * kColTileX + (threadIdx.x & 63)) < W && lcol >= 0 && lcol < W;
+  const int lcol = xt * kColTileX + lane + ox;
+  const bool lcol_ok = x < W && lcol >= 0 && lcol < W;
+  const float *Tc = Tr + (lcol_ok ? lcol : 0);
 
   unsigned kmax1 = 0u, kmax2 = 0u;
   unsigned long long amax = 0ull;
 
+  // Tile t holds source rows s_t + [0, nrows), s_t = 64 t + oy - ky.  Tile t+1
+  // shares rows [64, nrows) of tile t (moved to [0, Lp) in LDS) and needs 64
+  // new rows, fetched into registers while tile t is filtered (clamped
+  // addresses + select: the loads are unconditional).
+  auto src_row = [&](int row) -> float {
+    const bool in = lcol_ok && row >= 0 && row < H;
+    const float v = Tc[(long long)(in ? row : 0) * W];
+    return in ? v : 0.0f;
+  };
+  constexpr int kNew = kColTileY / kColWaves;  // new rows per thread
+  float pre[kNew];
+  for (int rr = wave; rr < nrows; rr += kColWaves) tile[rr * kColTileX + lane] = src_row(oy - ky + rr);
+  __syncthreads();
+
   for (int ty0 = 0; ty0 < H; ty0 += kColTileY) {
-    const int s0 = ty0 + oy - ky;  // source row of tap 0 for output row ty0
-    __syncthreads();
-    for (int rr = threadIdx.x >> 6; rr < nrows; rr += kColWaves) {
-      const int row = s0 + rr;
-      tile[rr * kColTileX + (threadIdx.x & 63)] =
-          (lcol_ok && row >= 0 && row < H) ? Tr[(long long)row * W + lcol] : 0.0f;
+    const bool more = ty0 + kColTileY < H;
+    if (more) {
+      const int s1 = ty0 + kColTileY + oy - ky + Lp;
+#pragma unroll
+      for (int k = 0; k < kNew; ++k) pre[k] = src_row(s1 + wave + kColWaves * k);
     }
-    __syncthreads();
 
     const float *tw = tile + wave * kColRows * kColTileX + lane;
     float acc[kColRows], buf[kColRows];
@@ -473,6 +508,27 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
           kmax2 = max(kmax2, f2key(v2));
         }
       }
+    }
+    if (more) {
+      // move rows [64, nrows) to [0, Lp), then store the prefetched rows
+      for (int c0 = 0; c0 < Lp; c0 += kColMove * kColWaves) {
+        float mv[kColMove];
+        __syncthreads();  // every wave is done with this tile
+#pragma unroll
+        for (int k = 0; k < kColMove; ++k) {
+          const int rr = c0 + wave + kColWaves * k;
+          if (rr < Lp) mv[k] = tile[(kColTileY + rr) * kColTileX + lane];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kColMove; ++k) {
+          const int rr = c0 + wave + kColWaves * k;
+          if (rr < Lp) tile[rr * kColTileX + lane] = mv[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kNew; ++k) tile[(Lp + wave + kColWaves * k) * kColTileX + lane] = pre[k];
+      __syncthreads();
     }
   }
   if (J.dst_kind == kDstUpNode) {
@@ -609,7 +665,7 @@ static bool rowpass_fixed(int R) { return R == 24; }
 size_t rowpass_lds_bytes(int R, int W, int kx) {
   RowGeom g = row_geom(W, kx);
   size_t p = (size_t)R * g.Pw;
-  if (!rowpass_fixed(R)) p += (size_t)R * 4 * g.Wq;  // the E buffer of the generic kernel
+  if (!rowpass_fixed(R)) p += (size_t)R * ((W + 3) & ~3);  // the E buffer of the generic kernel
   return sizeof(float) * p;
 }
 

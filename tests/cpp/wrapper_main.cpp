@@ -22,6 +22,7 @@ struct ExpParam {
   float max_object_scale() const { return max_scale; }
   unsigned num_scale_steps() const { return S; }
   float strip_border_detections() const { return strip; }
+  float roi_save_num_samples() const { return 1000.0f; }
 };
 struct PartDef {
   bool det, up, root;
@@ -122,9 +123,13 @@ int main(int argc, char **argv) {
   for (int p = 0; p < P; ++p)
     for (int s = 0; s < S; ++s) fwrite(marg[p][s].data(), sizeof(float), (size_t)R * H * W, o);
   for (int p = 0; p < P; ++p) {
-    int rec[4] = {best[p][0].scaleidx, best[p][0].rotidx, best[p][0].y, best[p][0].x};
-    fwrite(rec, sizeof(int), 4, o);
-    fwrite(&best[p][0].score, sizeof(float), 1, o);
+    int nh = (int)best[p].size();
+    fwrite(&nh, sizeof(int), 1, o);
+    for (const PartHyp &h : best[p]) {
+      int rec[4] = {h.scaleidx, h.rotidx, h.y, h.x};
+      fwrite(rec, sizeof(int), 4, o);
+      fwrite(&h.score, sizeof(float), 1, o);
+    }
   }
   for (int p = 0; p < P; ++p)
     for (int s = 0; s < S; ++s) fwrite(det[p][s].data(), sizeof(float), (size_t)R * H * W, o);

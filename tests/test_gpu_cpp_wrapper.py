@@ -55,10 +55,15 @@ def test_cpp_wrapper_matches_oracle(gpu, tmp_path):
     o += 4 * n_grid
     hyps = []
     for p in range(P):
-        rec = np.frombuffer(raw, np.int32, 4, o)
-        sc = np.frombuffer(raw, np.float32, 1, o + 16)[0]
-        hyps.append((tuple(int(v) for v in rec), float(sc)))
-        o += 20
+        nh = int(np.frombuffer(raw, np.int32, 1, o)[0])
+        o += 4
+        hp = []
+        for _ in range(nh):
+            rec = np.frombuffer(raw, np.int32, 4, o)
+            sc = np.frombuffer(raw, np.float32, 1, o + 16)[0]
+            hp.append((tuple(int(v) for v in rec), float(sc)))
+            o += 20
+        hyps.append(hp)
     masked = np.frombuffer(raw, np.float32, n_grid, o).reshape(P, S, R, H, W)
 
     r32, p32, b32, s32 = oracle.root_posterior_rot(oracle.exp_param_from(ep), model.is_detect, model.is_upright, 4,
@@ -67,8 +72,14 @@ def test_cpp_wrapper_matches_oracle(gpu, tmp_path):
         for p in range(P):
             assert oracle.rel_l1(oracle.normalized(marg[p, s]), oracle.normalized(p32[p, s])) <= 1e-5
     for p in range(P):
-        (scaleidx, rotidx, y, x), score = hyps[p]
+        (scaleidx, rotidx, y, x), score = hyps[p][0]
         assert (scaleidx, rotidx, y, x) == (S - 1,) + tuple(b32[S - 1, p])
+        # then the local maxima of the last scale's marginal, tagged scale 0
+        # (objectdetect_findrot.cpp:280-283, objectdetect_aux.cpp:304); on the
+        # float32 oracle's marginal where values agree to the last bit, else on ours
+        lm = oracle.find_local_max(marg[p, S - 1], 1000)
+        assert [h[0] for h in hyps[p][1:]] == [(0, int(d), int(y), int(x)) for d, x, y, v in lm]
+        assert len(hyps[p]) - 1 == len(lm) > 0
     finite = r32 > -1e5
     np.testing.assert_allclose(root[finite], r32[finite], rtol=2e-5, atol=2e-5)
     # masks written into the caller's grids as the reference does (:509-551)

@@ -239,3 +239,73 @@ def top2_gap(logb):
     m = b[i]
     b2 = np.delete(b, i)
     return float(m - b2.max())
+
+
+# ------------------------------------------------ off-diagonal covariance (f1)
+
+def _bind_f1():
+    L = lib()
+    if getattr(L, "_f1_bound", False):
+        return L
+    for t, p in (("f32", _fp), ("f64", _dp)):
+        getattr(L, "orc_gauss_filter2d_" + t).argtypes = [p, p, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+    L.orc_eig2d.argtypes = [_dp, _dp, _dp]
+    L.orc_transform_direct_f32.argtypes = [_fp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int, _dp,
+                                           ctypes.c_float]
+    L.orc_transform_bilinear_f32.argtypes = [_fp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int, _dp,
+                                             ctypes.c_float]
+    L.orc_ingest_scores_f32.argtypes = [_fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _fp]
+    L._f1_bound = True
+    return L
+
+
+def eig2d(C):
+    L = _bind_f1()
+    c = np.ascontiguousarray(np.asarray(C, dtype=np.float64).reshape(4))
+    V = np.empty(4)
+    E = np.empty(4)
+    L.orc_eig2d(_ptr(c), _ptr(V), _ptr(E))
+    return V.reshape(2, 2), E.reshape(2, 2)
+
+
+def gauss_filter2d(grid, C, is_sparse):
+    """gaussFilter2d: diagonal C, or gaussFilter2dOffset in C's eigenbasis."""
+    L = _bind_f1()
+    g = np.ascontiguousarray(grid)
+    out = np.empty_like(g)
+    c = np.ascontiguousarray(np.asarray(C, dtype=np.float64).reshape(4))
+    _chk(getattr(L, "orc_gauss_filter2d_" + _sfx(g.dtype))(_ptr(g), _ptr(out), g.shape[0], g.shape[1], _ptr(c),
+                                                            int(is_sparse)), "gauss_filter2d")
+    return out
+
+
+def transform_direct(grid, T31, out_shape, default_value=0.0):
+    L = _bind_f1()
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    out = np.empty(out_shape, dtype=np.float32)
+    T = np.ascontiguousarray(np.asarray(T31, dtype=np.float64).reshape(9))
+    L.orc_transform_direct_f32(_ptr(g), g.shape[0], g.shape[1], _ptr(out), out_shape[0], out_shape[1], _ptr(T),
+                               default_value)
+    return out
+
+
+def transform_bilinear(grid, T21, out_shape, default_value=0.0):
+    L = _bind_f1()
+    g = np.ascontiguousarray(grid, dtype=np.float32)
+    out = np.empty(out_shape, dtype=np.float32)
+    T = np.ascontiguousarray(np.asarray(T21, dtype=np.float64).reshape(9))
+    L.orc_transform_bilinear_f32(_ptr(g), g.shape[0], g.shape[1], _ptr(out), out_shape[0], out_shape[1], _ptr(T),
+                                 default_value)
+    return out
+
+
+def ingest_scores(score_grid, Tig, interpolate, H, W):
+    """loadScoreGrid's grid-to-image mapping, then clip_scores_fill + log."""
+    L = _bind_f1()
+    g = np.ascontiguousarray(score_grid, dtype=np.float32)
+    n, gh, gw = g.shape
+    T = np.ascontiguousarray(np.asarray(Tig, dtype=np.float64).reshape(n, 9))
+    out = np.empty((n, H, W), dtype=np.float32)
+    L.orc_ingest_scores_f32(_ptr(g), n, gh, gw, _ptr(T), int(interpolate), H, W, _ptr(out))
+    return out

@@ -246,6 +246,83 @@ static void transform_nearest(const T *in, T *out, int H, int W, const M3 &T21, 
     }
 }
 
+// multi_array_transform.hpp:122-239, TM_DIRECT: every source cell whose value
+// differs from default_value is written to the nearest output cell of its
+// image under T31 = T32 T21; later cells of the scan (x outer, y inner)
+// overwrite earlier ones.  adaptive: the output is the bounding box of the
+// mapped grid (T23 = translation(min corner)), else it keeps its size.
+template <class T>
+static void transform_direct(const T *in, int in_h, int in_w, T *out, int out_h, int out_w, const M3 &T31,
+                             T default_value) {
+  for (int x3 = 0; x3 < out_w; ++x3)
+    for (int y3 = 0; y3 < out_h; ++y3) out[(size_t)y3 * out_w + x3] = default_value;
+  for (int x1 = 0; x1 < in_w; ++x1)
+    for (int y1 = 0; y1 < in_h; ++y1) {
+      const T v = in[(size_t)y1 * in_w + x1];
+      if (v != default_value) {
+        double x3, y3;
+        map_point(T31, (double)x1, (double)y1, x3, y3);
+        int ix3 = (int)std::floor(x3 + 0.5);
+        int iy3 = (int)std::floor(y3 + 0.5);
+        if (ix3 >= 0 && ix3 < out_w && iy3 >= 0 && iy3 < out_h) out[(size_t)iy3 * out_w + ix3] = v;
+      }
+    }
+}
+
+// multi_array_transform.hpp:187-239, TM_BILINEAR: out(x3, y3) samples the
+// input at T13 (x3, y3); a sample within 10 float epsilons of a grid point
+// takes that point, a sample in the last row or column is left at the default
+template <class T>
+static void transform_bilinear(const T *in, int in_h, int in_w, T *out, int out_h, int out_w, const M3 &T13,
+                               T default_value) {
+  const float eps10 = 10 * std::numeric_limits<float>::epsilon();
+  for (int x3 = 0; x3 < out_w; ++x3)
+    for (int y3 = 0; y3 < out_h; ++y3) {
+      T r = default_value;
+      double x1, y1;
+      map_point(T13, (double)x3, (double)y3, x1, y1);
+      int ix1 = (int)std::floor(x1);
+      int iy1 = (int)std::floor(y1);
+      if (ix1 >= 0 && ix1 < in_w && iy1 >= 0 && iy1 < in_h) {
+        float a = x1 - ix1;
+        float b = y1 - iy1;
+        auto at = [&](int yy, int xx) { return in[(size_t)yy * in_w + xx]; };
+        if (a < eps10 && b < eps10)
+          r = at(iy1, ix1);
+        else if (ix1 < in_w - 1 && iy1 < in_h - 1)
+          r = (1.0f - b) * (1.0f - a) * at(iy1, ix1) + (1.0f - b) * a * at(iy1, ix1 + 1) +
+              b * (1.0f - a) * at(iy1 + 1, ix1) + b * a * at(iy1 + 1, ix1 + 1);
+      }
+      out[(size_t)y3 * out_w + x3] = r;
+    }
+}
+
+// hc::get_transformed_bbox (homogeneous_coord.cpp:128-146): images of the
+// four corner cells under T21
+static void transformed_bbox(const M3 &T21, int in_w, int in_h, double &minx, double &miny, double &maxx,
+                             double &maxy) {
+  const double cx[4] = {0.0, (double)(in_w - 1), 0.0, (double)(in_w - 1)};
+  const double cy[4] = {0.0, 0.0, (double)(in_h - 1), (double)(in_h - 1)};
+  for (int i = 0; i < 4; ++i) {
+    double p[3] = {cx[i], cy[i], 1.0}, q[3];
+    prod_vec(T21, p, q);
+    if (i == 0 || q[0] < minx) minx = q[0];
+    if (i == 0 || q[0] > maxx) maxx = q[0];
+    if (i == 0 || q[1] < miny) miny = q[1];
+    if (i == 0 || q[1] > maxy) maxy = q[1];
+  }
+}
+
+// objectdetect_aux.hpp:41-59 then multi_array_op.hpp:154-166: negative
+// scores become 1e-4, then log with 0 -> LOG_ZERO
+template <class T>
+static void clip_fill_log(T *p, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    if (p[i] < 0) p[i] = (T)0.0001;
+    p[i] = (p[i] == 0) ? (T)LOG_ZERO : (T)std::log(p[i]);
+  }
+}
+
 // multi_array_filter.hpp:115-155: out[s] = sum_j in[(s - n + j) mod n_in] f[j]
 // on a strided 1-D view.
 template <class T>
@@ -315,12 +392,95 @@ static int gauss_filter_diag2d(const T *in, T *out, int H, int W, const double C
   return ORC_OK;
 }
 
+// boost_math.cpp:40-99: C = V E V', E = diag(smallest, largest), V's first
+// column the eigenvector of the smallest eigenvalue
+static void eig2d(const double M[4], double V[4], double E[4]) {
+  const double m11 = M[0], m12 = M[1], m22 = M[3];
+  double e1, e2, v11, v21;
+  if (m12 != 0) {
+    double sqrtD = std::sqrt((m11 - m22) * (m11 - m22) + 4 * m12 * m12);
+    e1 = 0.5 * (m11 + m22 - sqrtD);
+    e2 = 0.5 * (m11 + m22 + sqrtD);
+    v11 = 0.5 * (m11 - m22 - sqrtD) / m12;
+    v21 = 1;
+  } else if (m11 < m22) {
+    e1 = m11;
+    e2 = m22;
+    v11 = 1;
+    v21 = 0;
+  } else {
+    e1 = m22;
+    e2 = m11;
+    v11 = 0;
+    v21 = 1;
+  }
+  double norm_v1 = std::sqrt(v11 * v11 + v21 * v21);
+  v11 /= norm_v1;
+  v21 /= norm_v1;
+  V[0] = v11;   // V(0,0)
+  V[1] = -v21;  // V(0,1) = v12
+  V[2] = v21;   // V(1,0)
+  V[3] = v11;   // V(1,1) = v22
+  E[0] = e1;
+  E[1] = 0;
+  E[2] = 0;
+  E[3] = e2;
+}
+
+// hc::get_homogeneous_matrix (homogeneous_coord.cpp:38-48)
+static M3 homogeneous(double r00, double r01, double r10, double r11, double dx, double dy) {
+  M3 T = zero3();
+  T.m[0][0] = r00;
+  T.m[0][1] = r01;
+  T.m[1][0] = r10;
+  T.m[1][1] = r11;
+  T.m[0][2] = dx;
+  T.m[1][2] = dy;
+  T.m[2][2] = 1;
+  return T;
+}
+
+// multi_array_filter.hpp:334-369 with offset 0 (gaussFilter2d, :384-387):
+// rotate into the eigenbasis of C (TM_DIRECT when sparse, else TM_BILINEAR,
+// onto the bounding box of the rotated grid, transform_grid_resize
+// multi_array_transform.hpp:284-305), filter with the diagonal eigenvalues,
+// rotate back bilinearly onto the input's size.  Padding value 0.
+template <class T>
+static int gauss_filter_offset2d(const T *in, T *out, int H, int W, const double C[4], bool bIsSparse) {
+  double V[4], E[4];
+  eig2d(C, V, E);
+  if (!(E[0] > 0 && E[3] > 0)) return ORC_ERR_COV;
+  M3 T21 = homogeneous(V[0], V[2], V[1], V[3], 0, 0);  // trans(V)
+  double minx, miny, maxx, maxy;
+  transformed_bbox(T21, W, H, minx, miny, maxx, maxy);
+  const int tw = (int)std::ceil(maxx - minx), th = (int)std::ceil(maxy - miny);
+  if (tw < 1 || th < 1) return ORC_ERR_ARG;
+  M3 T23 = hc_translation(minx, miny), T32 = hc_translation(-minx, -miny);
+  std::vector<T> tin((size_t)th * tw), tsm((size_t)th * tw);
+  if (bIsSparse)
+    transform_direct(in, H, W, tin.data(), th, tw, prod(T32, T21), (T)0);
+  else
+    transform_bilinear(in, H, W, tin.data(), th, tw, prod(hc_inverse(T21), T23), (T)0);
+  int st = gauss_filter_diag2d(tin.data(), tsm.data(), th, tw, E);
+  if (st != ORC_OK) return st;
+  M3 T42 = homogeneous(V[0], V[1], V[2], V[3], -0.0, -0.0);
+  M3 T43 = prod(T42, T23);
+  transform_bilinear(tsm.data(), th, tw, out, H, W, prod(hc_inverse(T43), ident3()), (T)0);
+  return ORC_OK;
+}
+
+// gaussFilter2d (multi_array_filter.hpp:374-388)
+template <class T>
+static int gauss_filter2d(const T *in, T *out, int H, int W, const double C[4], bool bIsSparse) {
+  if (C[1] == 0 && C[2] == 0) return gauss_filter_diag2d(in, out, H, W, C);
+  return gauss_filter_offset2d(in, out, H, W, C, bIsSparse);
+}
+
 // objectdetect_findrot.cpp:292-456 — computeRotJointMarginal
 template <class T>
 static int rot_joint_marginal(const orc_exp_param &ep, const T *child, T *parent, int R, int H, int W,
                               const double offset_c_10[2], const double offset_p_01[2], const double C[4],
                               double rot_mean, double rot_sigma, double scale, bool bIsSparse) {
-  (void)bIsSparse;  // only read by the non-diagonal path (gaussFilter2dOffset)
   const size_t plane = (size_t)H * W, n = (size_t)R * plane;
   const T in_M = grid_max(child, n);  // :300-302
 
@@ -363,8 +523,8 @@ static int rot_joint_marginal(const orc_exp_param &ep, const T *child, T *parent
   double scaleC[4];  // :422  square(scale)*C
   for (int i = 0; i < 4; ++i) scaleC[i] = (scale * scale) * C[i];
   for (int rotidx = 0; rotidx < R; ++rotidx) {  // :423-429 -> gaussFilter2d (multi_array_filter.hpp:374-388)
-    int st = gauss_filter_diag2d(rot_filter_result.data() + rotidx * plane, log_joint_01.data() + rotidx * plane, H,
-                                 W, scaleC);
+    int st = gauss_filter2d(rot_filter_result.data() + rotidx * plane, log_joint_01.data() + rotidx * plane, H, W,
+                            scaleC, bIsSparse);
     if (st != ORC_OK) return st;
   }
 
@@ -695,6 +855,45 @@ int orc_root_posterior_rot_f64(const orc_exp_param *ep, int nparts, const int *i
 }
 int orc_find_local_max_f32(const float *grid, int D, int H, int W, int max_n, double *out) {
   return find_local_max(grid, D, H, W, max_n, out);
+}
+int orc_gauss_filter2d_f32(const float *in, float *out, int H, int W, const double *C, int is_sparse) {
+  return gauss_filter2d(in, out, H, W, C, is_sparse != 0);
+}
+int orc_gauss_filter2d_f64(const double *in, double *out, int H, int W, const double *C, int is_sparse) {
+  return gauss_filter2d(in, out, H, W, C, is_sparse != 0);
+}
+int orc_eig2d(const double *C, double *V, double *E) {
+  eig2d(C, V, E);
+  return ORC_OK;
+}
+int orc_transform_direct_f32(const float *in, int in_h, int in_w, float *out, int out_h, int out_w,
+                             const double *T31, float default_value) {
+  M3 T;
+  std::memcpy(T.m, T31, sizeof(T.m));
+  transform_direct(in, in_h, in_w, out, out_h, out_w, T, default_value);
+  return ORC_OK;
+}
+int orc_transform_bilinear_f32(const float *in, int in_h, int in_w, float *out, int out_h, int out_w,
+                               const double *T21, float default_value) {
+  M3 T;
+  std::memcpy(T.m, T21, sizeof(T.m));
+  transform_bilinear(in, in_h, in_w, out, out_h, out_w, prod(hc_inverse(T), ident3()), default_value);
+  return ORC_OK;
+}
+int orc_ingest_scores_f32(const float *score_grid, int n, int gh, int gw, const double *Tig, int interpolate, int H,
+                          int W, float *out) {
+  for (int i = 0; i < n; ++i) {
+    M3 T;
+    std::memcpy(T.m, Tig + 9 * i, sizeof(T.m));
+    const float *src = score_grid + (size_t)i * gh * gw;
+    float *dst = out + (size_t)i * H * W;
+    if (interpolate)
+      transform_bilinear(src, gh, gw, dst, H, W, prod(hc_inverse(T), ident3()), 0.0f);
+    else
+      transform_direct(src, gh, gw, dst, H, W, prod(ident3(), T), 0.0f);
+    clip_fill_log(dst, (size_t)H * W);
+  }
+  return ORC_OK;
 }
 
 int orc_message_plan(const orc_exp_param *ep, int R, int H, int W, const double *offset_c, const double *offset_p,

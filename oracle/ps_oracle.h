@@ -95,6 +95,24 @@ int orc_root_posterior_rot_f64(const orc_exp_param *ep, int nparts, const int *i
  * max_n of them, each written as (d, x, y, value); returns the count. */
 int orc_find_local_max_f32(const float *grid, int D, int H, int W, int max_n, double *out);
 
+/* gaussFilter2d (multi_array_filter.hpp:374-388): diagonal C or the
+ * eigenbasis path gaussFilter2dOffset (:334-369) with offset 0 */
+int orc_gauss_filter2d_f32(const float *in, float *out, int H, int W, const double *C, int is_sparse);
+int orc_gauss_filter2d_f64(const double *in, double *out, int H, int W, const double *C, int is_sparse);
+/* boost_math.cpp:40-99; V, E row major */
+int orc_eig2d(const double *C, double *V, double *E);
+/* multi_array_transform.hpp:122-239: TM_DIRECT with T31 = T32 T21 (scatter),
+ * TM_BILINEAR of a fixed-size output with T21 (gather through its inverse) */
+int orc_transform_direct_f32(const float *in, int in_h, int in_w, float *out, int out_h, int out_w,
+                             const double *T31, float default_value);
+int orc_transform_bilinear_f32(const float *in, int in_h, int in_w, float *out, int out_h, int out_w,
+                               const double *T21, float default_value);
+/* PartApp::loadScoreGrid's grid-to-image mapping (partapp.cpp:866-899, Tig per
+ * slice, NO_CLASS_VALUE 0) followed by clip_scores_fill + computeLogGrid
+ * (objectdetect_findrot.cpp:834-845, objectdetect_aux.hpp:41-59) */
+int orc_ingest_scores_f32(const float *score_grid, int n, int gh, int gw, const double *Tig, int interpolate, int H,
+                          int W, float *out);
+
 /* The integer plan of one message, evaluated pixel by pixel with the
  * reference's own double arithmetic (objectdetect_findrot.cpp:316-359,437-448;
  * multi_array_transform.hpp:193-216).  For every rotation r the forward warp

@@ -434,6 +434,8 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   const float *dun = J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : nullptr;
   float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
   const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
+  const bool need_u = (J.dst_kind == kDstUpNode && J.dst_add_unary) || J.dst_kind == kDstDown;
+  const bool need_b = J.dst_kind == kDstDown && J.dst_base == 0;
   // column loaded by this thread's lane slot, and whether it exists
   const int lcol = xt * kColTileX + lane + ox;
   const bool lcol_ok = x < W && lcol >= 0 && lcol < W;
@@ -464,6 +466,25 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
       for (int k = 0; k < kNew; ++k) pre[k] = src_row(s1 + wave + kColWaves * k);
     }
 
+    // the epilogue's operands (unary, stored belief) are loaded now so that
+    // their latency hides behind the filter
+    const int y0 = ty0 + wave * kColRows;
+    float pu[kColRows], pb[kColRows];
+    if (need_u) {
+#pragma unroll
+      for (int i = 0; i < kColRows; ++i) {
+        const bool in = xin && y0 + i < H;
+        pu[i] = dun[in ? r * HW + (long long)(y0 + i) * W + x : 0];
+      }
+    }
+    if (need_b) {
+#pragma unroll
+      for (int i = 0; i < kColRows; ++i) {
+        const bool in = xin && y0 + i < H;
+        pb[i] = dst[in ? r * HW + (long long)(y0 + i) * W + x : 0];
+      }
+    }
+
     const float *tw = tile + wave * kColRows * kColTileX + lane;
     float acc[kColRows], buf[kColRows];
 #pragma unroll
@@ -477,7 +498,6 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
       col_chunk8<8>(acc, buf, fy, tw, t0 + 8);
     }
 
-    const int y0 = ty0 + wave * kColRows;
 #pragma unroll
     for (int i = 0; i < kColRows; ++i) {
       const int y = y0 + i;
@@ -491,13 +511,13 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
       } else if (J.dst_kind == kDstUpNode) {
         // belief of a chain node: 0 + message (+ unary), :637, :652-654
         float v = msg;
-        if (J.dst_add_unary) v = v + (xmask ? kLogZero : dun[e]);
+        if (J.dst_add_unary) v = v + (xmask ? kLogZero : pu[i]);
         dst[e] = v;
         kmax1 = max(kmax1, f2key(v));
       } else {
         // marginal = belief + downward message (:201); next source = unary + message (:220-222)
-        const float u = xmask ? kLogZero : dun[e];
-        const float base = J.dst_base == 0 ? dst[e] : (J.dst_base == 1 ? u : 0.0f);
+        const float u = xmask ? kLogZero : pu[i];
+        const float base = J.dst_base == 0 ? pb[i] : (J.dst_base == 1 ? u : 0.0f);
         const float mg = base + msg;
         dst[e] = mg;
         const unsigned long long k = amax_key(mg, (unsigned)e);

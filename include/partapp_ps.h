@@ -142,10 +142,12 @@ int ps_marginals(const ps_model *model, const ps_instance *instances, int n_inst
 
 /* computeRotJointMarginal (objectdetect_findrot.cpp:292-456; objectdetect.h:277-282):
  * one message child[R][H][W] -> parent[R][H][W] on the device, with the
- * model's exp_param.  Synchronous. */
+ * model's exp_param.  is_sparse is bIsSparse: with an off-diagonal C it
+ * selects the nearest scatter (else bilinear) rotation into the eigenbasis
+ * (multi_array_filter.hpp:351-355).  Synchronous. */
 int ps_message(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
                const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
-               double rot_sigma, double scale, void *stream);
+               double rot_sigma, double scale, int is_sparse, void *stream);
 
 /* findLocalMax (objectdetect_aux.cpp:193-251) on device grids.  Grid g is
  * [D][H][W] floats at grids_dev + offsets[g] (dims[3g..3g+2] = D, H, W; D is

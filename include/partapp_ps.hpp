@@ -280,7 +280,7 @@ void computeRotJointMarginal(const ExpParamT &exp_param, Grid3 &log_prob_child, 
   double cc[4] = {C(0, 0), C(0, 1), C(1, 0), C(1, 1)};
   int st = PS_OK;
   hipError_t e = hipMemcpy(c.p, log_prob_child.data(), n * sizeof(float), hipMemcpyHostToDevice);
-  if (e == hipSuccess) st = ps_message(m, c.f(), p.f(), H, W, oc, op, cc, rot_mean, rot_sigma, scale, nullptr);
+  if (e == hipSuccess) st = ps_message(m, c.f(), p.f(), H, W, oc, op, cc, rot_mean, rot_sigma, scale, bIsSparse ? 1 : 0, nullptr);
   if (e == hipSuccess && st == PS_OK)
     e = hipMemcpy(log_prob_parent.data(), p.p, n * sizeof(float), hipMemcpyDeviceToHost);
   ps_model_destroy(m);

@@ -112,7 +112,7 @@ SIGNATURES = {
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ps_map), ctypes.c_void_p]),
     "ps_message": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                   c_double_p, c_double_p, c_double_p, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_double, ctypes.c_void_p]),
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
     "ps_find_local_max": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), c_int_p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_last_error_string": (ctypes.c_char_p, []),

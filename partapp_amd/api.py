@@ -282,8 +282,9 @@ def compute_root_posterior_rot(model: PartModel, log_part_detections, joints: Op
 
 
 def compute_rot_joint_marginal(model: PartModel, log_prob_child, offset_c, offset_p, C, rot_mean, rot_sigma,
-                               scale, stream=None):
-    """computeRotJointMarginal: child [R][H][W] (torch CUDA float32) -> parent message."""
+                               scale, is_sparse=False, stream=None):
+    """computeRotJointMarginal (objectdetect_findrot.cpp:292-456): child [R][H][W]
+    (torch CUDA float32) -> parent message; is_sparse is bIsSparse."""
     import torch
     R, H, W = log_prob_child.shape
     child = log_prob_child.contiguous()
@@ -292,7 +293,8 @@ def compute_rot_joint_marginal(model: PartModel, log_prob_child, offset_c, offse
     op = (ctypes.c_double * 2)(*offset_p)
     cc = (ctypes.c_double * 4)(*C)
     check(lib().ps_message(model.handle, child.data_ptr(), parent.data_ptr(), H, W, oc, op, cc, float(rot_mean),
-                           float(rot_sigma), float(scale), _stream_handle(stream)), "ps_message")
+                           float(rot_sigma), float(scale), 1 if is_sparse else 0, _stream_handle(stream)),
+          "ps_message")
     return parent
 
 

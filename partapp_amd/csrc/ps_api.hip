@@ -28,6 +28,8 @@ hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 size_t colpass_lds_bytes(int ky);
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                           int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
+hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
+                          long long max_cells, long long max_hw, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
 }  // namespace ps
@@ -40,10 +42,14 @@ struct ps_model {
 
 namespace {
 
+// One level of messages: jobs [off, off + count), the first n_diag with a
+// diagonal covariance (row pass + column pass), the rest off-diagonal (row
+// pass + eigenbasis stages).
 struct Level {
-  int off = 0, count = 0, max_h = 0, max_w = 0;
+  int off = 0, count = 0, n_diag = 0, max_h = 0, max_w = 0;
   size_t lds = 0;      // row pass
   size_t col_lds = 0;  // column pass
+  long long od_cells = 0, od_hw = 0;  // max R*th*tw and H*W of the off-diagonal jobs
 };
 
 int hip_error(hipError_t e, const char *what) {
@@ -175,8 +181,29 @@ struct Builder {
     J.kx = mp.kx;
     J.ky = mp.ky;
     J.taps_r = add_taps(mp.rot_taps, 4);
-    J.taps_x = add_taps(mp.fx, 16);
-    J.taps_y = add_taps(mp.fy, 16);
+    if (mp.od) {
+      // the row pass keeps the rotation-filtered grid: kx = 0 and a unit tap
+      J.od = 1;
+      J.od_direct = mp.fwd_direct ? 1 : 0;
+      J.th = mp.th;
+      J.tw = mp.tw;
+      J.kxe = mp.kx;
+      J.kye = mp.ky;
+      J.taps_xe = add_taps(mp.fx, 16);
+      J.taps_ye = add_taps(mp.fy, 16);
+      J.kx = 0;
+      J.ky = 0;
+      J.taps_x = add_taps(std::vector<float>{1.0f}, 16);
+      J.taps_y = J.taps_x;
+      for (int i = 0; i < 6; ++i) {
+        J.t_fwd[i] = mp.t_fwd[i];
+        J.t_back[i] = mp.t_back[i];
+      }
+    } else {
+      J.od = 0;
+      J.taps_x = add_taps(mp.fx, 16);
+      J.taps_y = add_taps(mp.fy, 16);
+    }
     J.shifts = (int)shifts.size();
     shifts.insert(shifts.end(), mp.fwd_ox.begin(), mp.fwd_ox.end());
     shifts.insert(shifts.end(), mp.fwd_oy.begin(), mp.fwd_oy.end());
@@ -260,8 +287,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       const double scale = scale_from_index(m.ep, s);
       auto unary = [&](int p) { return gref(kBufUnary, in.unary_offset + ((long long)p * S + s) * n); };
       auto marg = [&](int p) { return gref(kBufMarginal, in.marginal_offset + ((long long)p * S + s) * n); };
-      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * n); };
-      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)(K + k) * n); };
+      const long long n64 = (n + 63) & ~63ll;  // grids start on 256-byte boundaries
+      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * n64); };
+      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)(K + k) * n64); };
       auto up_key = [&](int p) { return it * 2 * P + p; };
       auto dn_key = [&](int p) { return it * 2 * P + P + p; };
       auto slot = [&](int p) { return it * P + p; };
@@ -278,22 +306,52 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         mj.mask = part_mask(m, leaf, W);
         B.maxes.push_back(mj);
       }
+      // message plans of every edge: upward with the model's sparse flag
+      // (objectdetect_findrot.cpp:630-635), downward with offsets swapped,
+      // -rot_mean and bilinear rotations (:174-179, :224-229)
+      std::vector<std::vector<MessagePlan>> upp(K), dnp(K);
+      std::vector<long long> od_floats(K, 0);
+      for (int k = 0; k < K; ++k) {
+        const int len = (int)chains[k].size();
+        upp[k].resize(len);
+        dnp[k].resize(len);
+        for (int d = 1; d <= len; ++d) {
+          const int jidx = t.parent_joint[chains[k][d - 1]];
+          const ps_joint &jt = J[jidx];
+          const std::string where = "instance " + std::to_string(i) + ", joint " + std::to_string(jidx);
+          st = build_message_plan(m.ep, H, W, jt.offset_c, jt.offset_p, jt.C, jt.rot_mean, jt.rot_sigma, scale,
+                                  m.is_sparse != 0, upp[k][d - 1]);
+          if (st != PS_OK) return set_error(st, where + " (up): " + g_last_error);
+          st = build_message_plan(m.ep, H, W, jt.offset_p, jt.offset_c, jt.C, -jt.rot_mean, jt.rot_sigma, scale,
+                                  false, dnp[k][d - 1]);
+          if (st != PS_OK) return set_error(st, where + " (down): " + g_last_error);
+          for (const MessagePlan *mp : {&upp[k][d - 1], &dnp[k][d - 1]})
+            if (mp->od) od_floats[k] = std::max(od_floats[k], (od_scratch_floats(R, mp->th, mp->tw) + 63) & ~63ll);
+        }
+      }
+      // eigenbasis scratch of chain k, after the T grids and message slots
+      std::vector<long long> od_off(K, 0);
+      long long od_total = 0;
+      for (int k = 0; k < K; ++k) {
+        od_off[k] = (long long)scratch_off + 2ll * K * ((n + 63) & ~63ll) + od_total;
+        od_total += od_floats[k];
+      }
+      auto odbuf = [&](int k) { return gref(kBufScratch, od_off[k]); };
+
       // upward levels: the edge of chain k whose child has height l
       for (int k = 0; k < K; ++k) {
         const int len = (int)chains[k].size();
         for (int d = len; d >= 1; --d) {
           const int c = chains[k][d - 1];
           const int jidx = t.parent_joint[c];
-          const ps_joint &jt = J[jidx];
           const int p = m.joints[jidx].parent_idx;
-          MessagePlan mp;
-          st = build_message_plan(m.ep, H, W, jt.offset_c, jt.offset_p, jt.C, jt.rot_mean, jt.rot_sigma, scale, mp);
-          if (st != PS_OK) return set_error(st, "instance " + std::to_string(i) + ", joint " + std::to_string(jidx) + " (up): " + g_last_error);
+          const MessagePlan &mp = upp[k][d - 1];
           MsgJob mj;
           std::memset(&mj, 0, sizeof(mj));
           mj.H = H;
           mj.W = W;
           B.add_message(mp, mj);
+          mj.od_buf = mp.od ? odbuf(k) : kNone;
           if (t.height[c] == 1) {
             mj.src_kind = m.is_detect[c] ? kSrcUnary : kSrcZero;
             mj.src = unary(c);
@@ -352,17 +410,13 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         const int len = (int)chains[k].size();
         for (int d = 1; d <= len; ++d) {
           const int c = chains[k][d - 1];
-          const int jidx = t.parent_joint[c];
-          const ps_joint &jt = J[jidx];
-          MessagePlan mp;
-          // objectdetect_findrot.cpp:174-179, 224-229: offsets swapped, -rot_mean
-          st = build_message_plan(m.ep, H, W, jt.offset_p, jt.offset_c, jt.C, -jt.rot_mean, jt.rot_sigma, scale, mp);
-          if (st != PS_OK) return set_error(st, "instance " + std::to_string(i) + ", joint " + std::to_string(jidx) + " (down): " + g_last_error);
+          const MessagePlan &mp = dnp[k][d - 1];
           MsgJob mj;
           std::memset(&mj, 0, sizeof(mj));
           mj.H = H;
           mj.W = W;
           B.add_message(mp, mj);
+          mj.od_buf = mp.od ? odbuf(k) : kNone;
           mj.src_kind = kSrcGrid;
           mj.src = mbuf(k);
           mj.src_key = dn_key(c);
@@ -391,7 +445,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         mj.out = it * P + p;
         B.maps.push_back(mj);
       }
-      scratch_off += (size_t)2 * K * n;
+      scratch_off += (size_t)(2 * K * n64 + od_total);
     }
   }
 
@@ -407,6 +461,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->max_hw = max_hw;
   auto add_levels = [&](std::vector<std::vector<MsgJob>> &lv, std::vector<Level> &dst) -> int {
     for (auto &v : lv) {
+      std::stable_partition(v.begin(), v.end(), [](const MsgJob &j) { return j.od == 0; });
       Level L;
       L.off = (int)B.jobs.size();
       L.count = (int)v.size();
@@ -414,7 +469,13 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         L.max_h = std::max(L.max_h, j.H);
         L.max_w = std::max(L.max_w, j.W);
         L.lds = std::max(L.lds, rowpass_lds_bytes(R, j.W, j.kx));
-        L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
+        if (j.od) {
+          L.od_cells = std::max(L.od_cells, (long long)R * j.th * j.tw);
+          L.od_hw = std::max(L.od_hw, (long long)j.H * j.W);
+        } else {
+          ++L.n_diag;
+          L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
+        }
       }
       if (L.lds > kLdsLimit || L.col_lds > kLdsLimit)
         return set_error(PS_ERR_UNSUPPORTED, "row pass needs " + std::to_string(L.lds) +
@@ -555,8 +616,12 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     e = launch_rowpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
     profile_end(pp, ev2, s);
     if (e != hipSuccess) return;
+    // the column pass class also times the eigenbasis stages
     ev2 = profile_begin(pp, 3, s);
-    e = launch_colpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
+    e = launch_colpass(b, pl->jobs + L.off, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
+    if (e == hipSuccess)
+      e = launch_offdiag(b, pl->jobs + L.off + L.n_diag, L.count - L.n_diag, pl->taps, pl->shifts, pl->R, L.od_cells,
+                         L.od_hw, s);
     profile_end(pp, ev2, s);
   };
   for (const Level &L : pl->up) {
@@ -669,7 +734,7 @@ extern "C" int ps_marginals(const ps_model *model, const ps_instance *instances,
 
 extern "C" int ps_message(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
                           const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
-                          double rot_sigma, double scale, void *stream) {
+                          double rot_sigma, double scale, int is_sparse, void *stream) {
   if (!model || !child_dev || !parent_dev || !offset_c || !offset_p || !C)
     return set_error(PS_ERR_INVALID_ARG, "null argument");
   if (height < 1 || width < 1) return set_error(PS_ERR_INVALID_ARG, "sizes must be >= 1");
@@ -679,16 +744,17 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   int st = check_device(device);
   if (st != PS_OK) return st;
   MessagePlan mp;
-  st = build_message_plan(m.ep, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, mp);
+  st = build_message_plan(m.ep, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, is_sparse != 0, mp);
   if (st != PS_OK) return st;
-  size_t lds = rowpass_lds_bytes(R, width, mp.kx);
-  if (lds > kLdsLimit) return set_error(PS_ERR_UNSUPPORTED, "row pass LDS exceeds one workgroup");
   Builder B(m);
   MsgJob mj;
   std::memset(&mj, 0, sizeof(mj));
   mj.H = height;
   mj.W = width;
   B.add_message(mp, mj);
+  const size_t lds = rowpass_lds_bytes(R, width, mj.kx);
+  if (lds > kLdsLimit || (!mp.od && colpass_lds_bytes(mp.ky) > kLdsLimit))
+    return set_error(PS_ERR_UNSUPPORTED, "message needs more LDS than one workgroup has");
   mj.src_kind = kSrcUnary;  // a plain grid, read through the (empty) mask path
   mj.src = gref(kBufUnary, 0);
   mj.src_key = 0;
@@ -697,6 +763,8 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   mj.dst = gref(kBufMarginal, 0);
   mj.dst_unary = kNone;
   mj.dst2 = kNone;
+  const long long n64 = ((long long)R * height * width + 63) & ~63ll;
+  mj.od_buf = mp.od ? gref(kBufScratch, n64) : kNone;
   MaxJob xj;
   std::memset(&xj, 0, sizeof(xj));
   xj.H = height;
@@ -711,7 +779,7 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   size_t o_key = align(o_sh + B.shifts.size() * sizeof(int));
   size_t o_mk = align(o_key + sizeof(unsigned));
   size_t o_T = align(o_mk + sizeof(unsigned long long));
-  size_t total = o_T + sizeof(float) * (size_t)R * height * width;
+  size_t total = o_T + sizeof(float) * (size_t)(n64 + (mp.od ? od_scratch_floats(R, mp.th, mp.tw) : 0));
   void *mem = nullptr;
   hipError_t e = hipMalloc(&mem, total);
   if (e != hipSuccess) return set_error(PS_ERR_OUT_OF_MEMORY, "message scratch");
@@ -737,7 +805,12 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   if (e == hipSuccess) e = launch_reset(b.max_keys, 1, b.map_keys, 1, s);
   if (e == hipSuccess) e = launch_max(b, reinterpret_cast<const MaxJob *>(base + o_max), 1, R, height * width, s);
   if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, R, height, width, lds, s);
-  if (e == hipSuccess) e = launch_colpass(b, jd, 1, td, sd, R, height, width, colpass_lds_bytes(mp.ky), s);
+  if (e == hipSuccess) {
+    if (mp.od)
+      e = launch_offdiag(b, jd, 1, td, sd, R, (long long)R * mp.th * mp.tw, (long long)height * width, s);
+    else
+      e = launch_colpass(b, jd, 1, td, sd, R, height, width, colpass_lds_bytes(mp.ky), s);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(mem);
   if (e != hipSuccess) return hip_error(e, "message");
@@ -761,7 +834,7 @@ extern "C" int ps_debug_message_plan(const ps_exp_param *ep, int height, int wid
   if (ep->num_rotation_steps < 1 || ep->num_rotation_steps > (unsigned)kMaxRot)
     return set_error(PS_ERR_UNSUPPORTED, "num_rotation_steps must be in [1, 64]");
   MessagePlan mp;
-  int st = build_message_plan(*ep, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, mp);
+  int st = build_message_plan(*ep, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, false, mp);
   if (st != PS_OK) return st;
   const int R = (int)ep->num_rotation_steps;
   *delta = mp.delta;

@@ -1,0 +1,115 @@
+// Device helpers shared by the sweep kernels (ps_kernels.hip) and the
+// eigenbasis path of off-diagonal covariances (ps_offdiag.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ps_internal.hpp"
+
+namespace ps {
+
+// order-preserving 32-bit key of a float (larger float, larger key)
+__device__ __forceinline__ unsigned f2key(float f) {
+  unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(unsigned k) {
+  unsigned u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+// argmax key: larger score wins, then the smaller flat index (the first
+// maximum of a C-order scan, objectdetect_findrot.cpp:261-266)
+__device__ __forceinline__ unsigned long long amax_key(float v, unsigned idx) {
+  return ((unsigned long long)f2key(v) << 32) | (unsigned long long)(0xffffffffu - idx);
+}
+
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Base pointer of a GridRef.  Written as selects on the (uniform) buffer id
+// so the kernel argument block is never copied to scratch for indexing.
+__device__ __forceinline__ float *base_of(const Bases &b, int buf) {
+  float *p = buf == kBufMarginal ? b.marginal : b.scratch;
+  p = buf == kBufUnary ? const_cast<float *>(b.unary) : p;
+  p = buf == kBufRoot ? b.root : p;
+  return p;
+}
+__device__ __forceinline__ const float *cptr(const Bases &b, const GridRef &g) { return base_of(b, g.buf) + g.off; }
+__device__ __forceinline__ float *mptr(const Bases &b, const GridRef &g) { return base_of(b, g.buf) + g.off; }
+
+__device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) {
+  return ((m.rot_bits >> r) & 1ull) || x < m.strip || x >= W - m.strip;
+}
+
+// log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166)
+__device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
+
+// Reductions a message's epilogue feeds: the max of the next upward source,
+// the argmax of a marginal and the max of the next downward source.
+struct DstAcc {
+  unsigned kmax1 = 0u, kmax2 = 0u;
+  unsigned long long amax = 0ull;
+};
+
+// The caller's accumulation of one message value msg at element
+// e = r*H*W + y*W + x of slice r.  pu is the unary of the part written (read
+// when need_u), pb the stored upward belief (read when need_b), xmask the
+// part's mask at (r, x).
+//   up, parent = root:  the message slot (objectdetect_findrot.cpp:637-649)
+//   up, chain node:     belief = 0 + message (+ unary) (:637, :652-654)
+//   down:               marginal = belief + message (:201), next downward
+//                       source = unary + message (:220-222)
+__device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *dst2, long long e, bool xmask,
+                                          float msg, float pu, float pb, DstAcc &a) {
+  if (J.dst_kind == kDstUpRoot) {
+    dst[e] = msg;
+  } else if (J.dst_kind == kDstUpNode) {
+    float v = msg;
+    if (J.dst_add_unary) v = v + (xmask ? kLogZero : pu);
+    dst[e] = v;
+    a.kmax1 = max(a.kmax1, f2key(v));
+  } else {
+    const float u = xmask ? kLogZero : pu;
+    const float base = J.dst_base == 0 ? pb : (J.dst_base == 1 ? u : 0.0f);
+    const float mg = base + msg;
+    dst[e] = mg;
+    const unsigned long long k = amax_key(mg, (unsigned)e);
+    a.amax = k > a.amax ? k : a.amax;
+    if (dst2) {
+      const float v2 = u + msg;
+      dst2[e] = v2;
+      a.kmax2 = max(a.kmax2, f2key(v2));
+    }
+  }
+}
+
+// Wave reductions of DstAcc and one atomic per wave; every lane of the wave
+// calls it.
+__device__ __forceinline__ void dst_flush(const Bases &b, const MsgJob &J, bool has_dst2, DstAcc &a, int lane) {
+  if (J.dst_kind == kDstUpNode) {
+    const unsigned v = wave_max_u32(a.kmax1);
+    if (lane == 0 && v) atomicMax(b.max_keys + J.dst_key, v);
+  } else if (J.dst_kind == kDstDown) {
+    const unsigned long long m = wave_max_u64(a.amax);
+    if (lane == 0 && m) atomicMax(b.map_keys + J.map_slot, m);
+    if (has_dst2) {
+      const unsigned v = wave_max_u32(a.kmax2);
+      if (lane == 0 && v) atomicMax(b.max_keys + J.dst2_key, v);
+    }
+  }
+}
+
+}  // namespace ps

@@ -74,7 +74,24 @@ struct MsgJob {
   GridRef dst2;       // Down with a grandchild: unary + message (next source), else buf -1
   int dst2_key;
   int map_slot;       // Down: argmax key of the child marginal
+  // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
+  // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
+  // unit x tap); each slice is then rotated onto a th x tw grid in the
+  // eigenbasis of scale^2 C, filtered there with the eigenvalue taps and
+  // sampled back bilinearly.
+  int od;          // 1: this message takes the eigenbasis path
+  int od_direct;   // forward rotation: 1 nearest scatter (TM_DIRECT), 0 bilinear gather
+  int th, tw;      // eigenbasis grid
+  int kxe, kye;    // half widths of the eigenbasis filters
+  int taps_xe;     // offsets into the tap pool
+  int taps_ye;
+  GridRef od_buf;  // floats: A[R][th][tw] (scatter keys: 2 floats per cell), then X[R][th][tw]
+  double t_fwd[6];   // rows 0-1; direct: grid -> eigenbasis, bilinear: eigenbasis -> grid
+  double t_back[6];  // rows 0-1; grid pixel -> eigenbasis sample position
 };
+
+// floats of the eigenbasis scratch of one message
+inline long long od_scratch_floats(int R, int th, int tw) { return 3ll * R * th * tw; }
 
 // Root of one (instance, scale): sums the upward messages into the root
 // belief, forms the "all but one child" root messages

@@ -24,58 +24,11 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "ps_device.hpp"
 #include "ps_internal.hpp"
 #include "../../include/partapp_ps.h"
 
 namespace ps {
-
-// ----------------------------------------------------------------- helpers
-
-__device__ __forceinline__ unsigned f2key(float f) {
-  unsigned u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float key2f(unsigned k) {
-  unsigned u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
-  return __uint_as_float(u);
-}
-// argmax key: larger score wins, then the smaller flat index (the first
-// maximum of a C-order scan, objectdetect_findrot.cpp:261-266)
-__device__ __forceinline__ unsigned long long amax_key(float v, unsigned idx) {
-  return ((unsigned long long)f2key(v) << 32) | (unsigned long long)(0xffffffffu - idx);
-}
-
-__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));
-  return v;
-}
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-
-// Base pointer of a GridRef.  Written as selects on the (uniform) buffer id
-// so the kernel argument block is never copied to scratch for indexing.
-__device__ __forceinline__ float *base_of(const Bases &b, int buf) {
-  float *p = buf == kBufMarginal ? b.marginal : b.scratch;
-  p = buf == kBufUnary ? const_cast<float *>(b.unary) : p;
-  p = buf == kBufRoot ? b.root : p;
-  return p;
-}
-__device__ __forceinline__ const float *cptr(const Bases &b, const GridRef &g) { return base_of(b, g.buf) + g.off; }
-__device__ __forceinline__ float *mptr(const Bases &b, const GridRef &g) { return base_of(b, g.buf) + g.off; }
-
-__device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) {
-  return ((m.rot_bits >> r) & 1ull) || x < m.strip || x >= W - m.strip;
-}
-
-// log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166)
-__device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
 
 // --------------------------------------------------------------- k_reset
 
@@ -441,8 +394,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   const bool lcol_ok = x < W && lcol >= 0 && lcol < W;
   const float *Tc = Tr + (lcol_ok ? lcol : 0);
 
-  unsigned kmax1 = 0u, kmax2 = 0u;
-  unsigned long long amax = 0ull;
+  DstAcc da;
 
   // Tile t holds source rows s_t + [0, nrows), s_t = 64 t + oy - ky.  Tile t+1
   // shares rows [64, nrows) of tile t (moved to [0, Lp) in LDS) and needs 64
@@ -506,28 +458,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
       // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
       const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(acc[i]) + M : kLogZero;
       const long long e = r * HW + (long long)y * W + x;
-      if (J.dst_kind == kDstUpRoot) {
-        dst[e] = msg;
-      } else if (J.dst_kind == kDstUpNode) {
-        // belief of a chain node: 0 + message (+ unary), :637, :652-654
-        float v = msg;
-        if (J.dst_add_unary) v = v + (xmask ? kLogZero : pu[i]);
-        dst[e] = v;
-        kmax1 = max(kmax1, f2key(v));
-      } else {
-        // marginal = belief + downward message (:201); next source = unary + message (:220-222)
-        const float u = xmask ? kLogZero : pu[i];
-        const float base = J.dst_base == 0 ? pb[i] : (J.dst_base == 1 ? u : 0.0f);
-        const float mg = base + msg;
-        dst[e] = mg;
-        const unsigned long long k = amax_key(mg, (unsigned)e);
-        amax = k > amax ? k : amax;
-        if (dst2) {
-          const float v2 = u + msg;
-          dst2[e] = v2;
-          kmax2 = max(kmax2, f2key(v2));
-        }
-      }
+      dst_store(J, dst, dst2, e, xmask, msg, need_u ? pu[i] : 0.0f, need_b ? pb[i] : 0.0f, da);
     }
     if (more) {
       // move rows [64, nrows) to [0, Lp), then store the prefetched rows
@@ -551,17 +482,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
       __syncthreads();
     }
   }
-  if (J.dst_kind == kDstUpNode) {
-    kmax1 = wave_max_u32(kmax1);
-    if (lane == 0 && kmax1) atomicMax(b.max_keys + J.dst_key, kmax1);
-  } else if (J.dst_kind == kDstDown) {
-    amax = wave_max_u64(amax);
-    if (lane == 0 && amax) atomicMax(b.map_keys + J.map_slot, amax);
-    if (dst2) {
-      kmax2 = wave_max_u32(kmax2);
-      if (lane == 0 && kmax2) atomicMax(b.max_keys + J.dst2_key, kmax2);
-    }
-  }
+  dst_flush(b, J, dst2 != nullptr, da, lane);
 }
 
 // ----------------------------------------------------------------- k_root

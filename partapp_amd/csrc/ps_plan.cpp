@@ -95,8 +95,154 @@ static bool uniform_offset(double shift, int n, int &off) {
   return true;
 }
 
+// ---- 3x3 homogeneous matrices for the eigenbasis transforms, row major.
+// Products sum from zero in index order and the inverse follows
+// homogeneous_coord.cpp:51-71, so the coordinates match the reference's
+// double arithmetic.
+namespace {
+struct Mat3 {
+  double m[3][3];
+};
+Mat3 mat_zero() {
+  Mat3 r;
+  for (auto &row : r.m)
+    for (double &v : row) v = 0.0;
+  return r;
+}
+Mat3 mat_ident() {
+  Mat3 r = mat_zero();
+  r.m[0][0] = r.m[1][1] = r.m[2][2] = 1.0;
+  return r;
+}
+Mat3 mat_mul(const Mat3 &a, const Mat3 &b) {
+  Mat3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) t += a.m[i][k] * b.m[k][j];
+      r.m[i][j] = t;
+    }
+  return r;
+}
+Mat3 mat_inverse(const Mat3 &T) {
+  Mat3 r;
+  const double D = T.m[0][0] * T.m[1][1] - T.m[1][0] * T.m[0][1];
+  r.m[0][0] = T.m[1][1] / D;
+  r.m[0][1] = -T.m[0][1] / D;
+  r.m[1][0] = -T.m[1][0] / D;
+  r.m[1][1] = T.m[0][0] / D;
+  for (int i = 0; i < 2; ++i) {
+    double t = 0.0;
+    for (int k = 0; k < 2; ++k) t += r.m[i][k] * T.m[k][2];
+    r.m[i][2] = -t;
+  }
+  r.m[2][0] = 0;
+  r.m[2][1] = 0;
+  r.m[2][2] = 1;
+  return r;
+}
+Mat3 mat_homogeneous(double r00, double r01, double r10, double r11, double dx, double dy) {
+  Mat3 T = mat_zero();
+  T.m[0][0] = r00;
+  T.m[0][1] = r01;
+  T.m[1][0] = r10;
+  T.m[1][1] = r11;
+  T.m[0][2] = dx;
+  T.m[1][2] = dy;
+  T.m[2][2] = 1;
+  return T;
+}
+Mat3 mat_translation(double dx, double dy) { return mat_homogeneous(1, 0, 0, 1, dx, dy); }
+void mat_store(const Mat3 &M, double out[9]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = M.m[i][j];
+}
+
+// boost_math.cpp:40-99: eigenvalues ascending; V's first column belongs to
+// the smallest, V = [v1 v2] with v2 = (-v21, v11)
+void eigen_2x2(const double M[4], double V[4], double E[2]) {
+  const double m11 = M[0], m12 = M[1], m22 = M[3];
+  double e1, e2, v11, v21;
+  if (m12 != 0) {
+    const double sqrtD = std::sqrt((m11 - m22) * (m11 - m22) + 4 * m12 * m12);
+    e1 = 0.5 * (m11 + m22 - sqrtD);
+    e2 = 0.5 * (m11 + m22 + sqrtD);
+    v11 = 0.5 * (m11 - m22 - sqrtD) / m12;
+    v21 = 1;
+  } else if (m11 < m22) {
+    e1 = m11;
+    e2 = m22;
+    v11 = 1;
+    v21 = 0;
+  } else {
+    e1 = m22;
+    e2 = m11;
+    v11 = 0;
+    v21 = 1;
+  }
+  const double norm_v1 = std::sqrt(v11 * v11 + v21 * v21);
+  v11 /= norm_v1;
+  v21 /= norm_v1;
+  V[0] = v11;
+  V[1] = -v21;
+  V[2] = v21;
+  V[3] = v11;
+  E[0] = e1;
+  E[1] = e2;
+}
+}  // namespace
+
+// The eigenbasis path of one message: multi_array_filter.hpp:334-369 with
+// offset 0, transform_grid_resize (multi_array_transform.hpp:284-305) and
+// get_transformed_bbox (homogeneous_coord.cpp:128-146).
+static int plan_offdiagonal(const double sC[4], int H, int W, bool is_sparse, MessagePlan &out) {
+  double V[4], E[2];
+  eigen_2x2(sC, V, E);
+  if (!(E[0] > 0 && E[1] > 0)) return set_error(PS_ERR_COVARIANCE, "covariance must be positive definite");
+  const Mat3 T21 = mat_homogeneous(V[0], V[2], V[1], V[3], 0, 0);  // V transposed
+  const double cx[4] = {0.0, (double)(W - 1), 0.0, (double)(W - 1)};
+  const double cy[4] = {0.0, 0.0, (double)(H - 1), (double)(H - 1)};
+  double minx = 0, maxx = 0, miny = 0, maxy = 0;
+  for (int i = 0; i < 4; ++i) {
+    const double p[3] = {cx[i], cy[i], 1.0};
+    double q[2];
+    for (int r = 0; r < 2; ++r) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) t += T21.m[r][k] * p[k];
+      q[r] = t;
+    }
+    if (i == 0 || q[0] < minx) minx = q[0];
+    if (i == 0 || q[0] > maxx) maxx = q[0];
+    if (i == 0 || q[1] < miny) miny = q[1];
+    if (i == 0 || q[1] > maxy) maxy = q[1];
+  }
+  out.tw = (int)std::ceil(maxx - minx);
+  out.th = (int)std::ceil(maxy - miny);
+  if (out.tw < 1 || out.th < 1) return set_error(PS_ERR_INVALID_ARG, "degenerate eigenbasis grid");
+  const Mat3 T23 = mat_translation(minx, miny), T32 = mat_translation(-minx, -miny);
+  out.fwd_direct = is_sparse;
+  if (is_sparse)
+    mat_store(mat_mul(T32, T21), out.t_fwd);
+  else
+    mat_store(mat_mul(mat_inverse(T21), T23), out.t_fwd);
+  const Mat3 T42 = mat_homogeneous(V[0], V[1], V[2], V[3], -0.0, -0.0);
+  mat_store(mat_mul(mat_inverse(mat_mul(T42, T23)), mat_ident()), out.t_back);
+  std::vector<double> fx, fy;
+  gaussian_taps(std::sqrt(E[0]), fx);
+  gaussian_taps(std::sqrt(E[1]), fy);
+  if ((int)fx.size() >= kMaxTaps || (int)fy.size() >= kMaxTaps)
+    return set_error(PS_ERR_KERNEL_SIZE, "spatial kernel must have fewer than 1000 taps");
+  out.fx.assign(fx.begin(), fx.end());
+  out.fy.assign(fy.begin(), fy.end());
+  out.kx = ((int)fx.size() - 1) / 2;
+  out.ky = ((int)fy.size() - 1) / 2;
+  out.od = true;
+  return PS_OK;
+}
+
 int build_message_plan(const ps_exp_param &ep, int H, int W, const double offset_c[2], const double offset_p[2],
-                       const double C[4], double rot_mean, double rot_sigma, double scale, MessagePlan &out) {
+                       const double C[4], double rot_mean, double rot_sigma, double scale, bool is_sparse,
+                       MessagePlan &out) {
   const int R = (int)ep.num_rotation_steps;
   // objectdetect_findrot.cpp:316-323
   double rot_step_size = (ep.max_part_rotation - ep.min_part_rotation) / ep.num_rotation_steps;
@@ -132,18 +278,22 @@ int build_message_plan(const ps_exp_param &ep, int H, int W, const double offset
   // spatial kernel: gaussFilter2d(scale^2 C) (:422-429, multi_array_filter.hpp:374-388)
   double sC[4];
   for (int i = 0; i < 4; ++i) sC[i] = (scale * scale) * C[i];
-  if (!(sC[1] == 0 && sC[2] == 0))
-    return set_error(PS_ERR_UNSUPPORTED, "off-diagonal covariance (gaussFilter2dOffset) is not supported");
-  if (!(sC[0] > 0 && sC[3] > 0)) return set_error(PS_ERR_COVARIANCE, "covariance diagonal must be > 0");
-  std::vector<double> fx, fy;
-  gaussian_taps(std::sqrt(sC[0]), fx);
-  gaussian_taps(std::sqrt(sC[3]), fy);
-  if ((int)fx.size() >= kMaxTaps || (int)fy.size() >= kMaxTaps)
-    return set_error(PS_ERR_KERNEL_SIZE, "spatial kernel must have fewer than 1000 taps");
-  out.fx.assign(fx.begin(), fx.end());
-  out.fy.assign(fy.begin(), fy.end());
-  out.kx = ((int)fx.size() - 1) / 2;
-  out.ky = ((int)fy.size() - 1) / 2;
+  out.od = false;
+  if (!(sC[1] == 0 && sC[2] == 0)) {
+    int st = plan_offdiagonal(sC, H, W, is_sparse, out);
+    if (st != PS_OK) return st;
+  } else {
+    if (!(sC[0] > 0 && sC[3] > 0)) return set_error(PS_ERR_COVARIANCE, "covariance diagonal must be > 0");
+    std::vector<double> fx, fy;
+    gaussian_taps(std::sqrt(sC[0]), fx);
+    gaussian_taps(std::sqrt(sC[3]), fy);
+    if ((int)fx.size() >= kMaxTaps || (int)fy.size() >= kMaxTaps)
+      return set_error(PS_ERR_KERNEL_SIZE, "spatial kernel must have fewer than 1000 taps");
+    out.fx.assign(fx.begin(), fx.end());
+    out.fy.assign(fy.begin(), fy.end());
+    out.kx = ((int)fx.size() - 1) / 2;
+    out.ky = ((int)fy.size() - 1) / 2;
+  }
 
   out.fwd_ox.assign(R, 0);
   out.fwd_oy.assign(R, 0);

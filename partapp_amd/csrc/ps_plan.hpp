@@ -44,6 +44,14 @@ struct MessagePlan {
   int kx = 0, ky = 0;
   std::vector<int> fwd_ox, fwd_oy;   // [R] by source slice
   std::vector<int> inv_ox, inv_oy;   // [R] by slice
+  // off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:334-369):
+  // rotate into the eigenbasis of scale^2 C onto a th x tw grid, filter
+  // there with fx, fy (kx, ky above are the eigenbasis half widths), sample back
+  bool od = false;
+  int th = 0, tw = 0;
+  bool fwd_direct = false;  // TM_DIRECT scatter (sparse) or TM_BILINEAR gather
+  double t_fwd[9] = {};     // direct: source -> eigenbasis grid; bilinear: grid -> source
+  double t_back[9] = {};    // output pixel -> sample position in the eigenbasis grid
 };
 
 // Status + message helpers; every failure path sets the thread's error text.
@@ -53,7 +61,8 @@ int build_model(const ps_model_desc *desc, Model &m);
 // joints must be P-1 joints with the model's topology
 int check_joints(const Model &m, const ps_joint *joints);
 int build_message_plan(const ps_exp_param &ep, int H, int W, const double offset_c[2], const double offset_p[2],
-                       const double C[4], double rot_mean, double rot_sigma, double scale, MessagePlan &out);
+                       const double C[4], double rot_mean, double rot_sigma, double scale, bool is_sparse,
+                       MessagePlan &out);
 
 double rot_from_index(const ps_exp_param &ep, int rotidx);
 double scale_from_index(const ps_exp_param &ep, int scaleidx);

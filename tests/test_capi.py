@@ -123,7 +123,12 @@ def test_plan_matches_oracle_pixel_evaluation(seed):
 def test_plan_error_codes():
     ep = ExpParam(num_rotation_steps=24)
     assert _debug_plan(ep, 10, 10, [0, 0], [0, 0], [0.0, 0, 0, 1.0], 0, 0.1, 1.0)[0] == -3   # C00 <= 0
-    assert _debug_plan(ep, 10, 10, [0, 0], [0, 0], [1.0, 0.2, 0.2, 1.0], 0, 0.1, 1.0)[0] == -5  # off-diagonal
+    # off-diagonal C: the filter runs in its eigenbasis, which must be positive definite
+    assert _debug_plan(ep, 10, 10, [0, 0], [0, 0], [1.0, 2.0, 2.0, 1.0], 0, 0.1, 1.0)[0] == -3
+    st, p = _debug_plan(ep, 10, 10, [0, 0], [0, 0], [5.0, 2.0, 2.0, 3.0], 0, 0.1, 1.5)
+    e_small, e_large = np.linalg.eigvalsh(np.array([[5.0, 2.0], [2.0, 3.0]]) * 1.5 ** 2)
+    assert st == 0
+    assert p["kx"] == math.floor(3 * math.sqrt(e_small) + 0.5) and p["ky"] == math.floor(3 * math.sqrt(e_large) + 0.5)
     assert _debug_plan(ep, 10, 10, [0, 0], [0, 0], [400.0 ** 2, 0, 0, 1.0], 0, 0.1, 1.0)[0] == -4  # >= 1000 taps
     assert _debug_plan(ExpParam(num_rotation_steps=1), 10, 10, [0, 0], [0, 0], [1.0, 0, 0, 1.0], 0, 0.1,
                        1.0)[0] == -4  # R = 1 cannot hold a clipped rotation kernel

@@ -169,6 +169,18 @@ typedef struct ps_local_max {
 int ps_find_local_max(const float *grids_dev, const long long *offsets, const int *dims, int n_grids, int max_n,
                       ps_local_max *out_dev, int *count_dev, void *stream);
 
+/* Unary ingestion (PartApp::loadScoreGrid, partapp.cpp:866-899, followed by
+ * clip_scores_fill + computeLogGrid, objectdetect_findrot.cpp:834-845):
+ * n score-grid slices [n][grid_height][grid_width] on the device, each mapped
+ * to image coordinates with its homogeneous 3x3 Tig (host, [n][9], row
+ * major: image = Tig * grid) by nearest scatter (interpolate = 0, TM_DIRECT)
+ * or bilinear gather (TM_BILINEAR), with NO_CLASS_VALUE 0 where nothing
+ * lands; then scores < 0 become 1e-4 and the log is taken (0 -> LOG_ZERO).
+ * Writes log_unary_dev [n][height][width].  Allocates a workspace (8 bytes
+ * per output cell for the scatter); synchronous. */
+int ps_ingest_scores(const float *score_grid_dev, int n, int grid_height, int grid_width, const double *Tig,
+                     int interpolate, int height, int width, float *log_unary_dev, void *stream);
+
 /* --------------------------------------------------------------- utilities */
 const char *ps_last_error_string(void);
 const char *ps_version(void);

@@ -242,6 +242,77 @@ void computeRootPosteriorRot(const PartAppT &part_app, std::vector<std::vector<G
   }
 }
 
+// findLocalMax of a [nScales][H][W] root posterior into a HypothesisList
+// (objectdetect_aux.cpp:266-286): one hypothesis per local maximum, appended,
+// with scale = scale_from_index(scale index), x, y and score.  HypListT needs
+// add_hyp() returning a pointer with set_scale / set_x / set_y / set_score
+// (the protobuf HypothesisList of HypothesisList.proto qualifies).
+template <class ExpParamT, class Grid3, class HypListT>
+void findLocalMax(const ExpParamT &exp_param, const Grid3 &log_prob_grid, HypListT &hypothesis_list,
+                  int max_hypothesis_number) {
+  const int D = (int)log_prob_grid.shape()[0], H = (int)log_prob_grid.shape()[1], W = (int)log_prob_grid.shape()[2];
+  if (max_hypothesis_number < 1) return;
+  const size_t n = (size_t)D * H * W;
+  DeviceBuffer g(n * sizeof(float)), lmd(sizeof(ps_local_max) * (size_t)max_hypothesis_number), cnd(sizeof(int));
+  check_hip(hipMemcpy(g.p, log_prob_grid.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  long long off = 0;
+  int dims[3] = {D, H, W};
+  check(ps_find_local_max(g.f(), &off, dims, 1, max_hypothesis_number, static_cast<ps_local_max *>(lmd.p),
+                          static_cast<int *>(cnd.p), nullptr),
+        "ps_find_local_max");
+  int count = 0;
+  check_hip(hipMemcpy(&count, cnd.p, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  std::vector<ps_local_max> lm((size_t)count);
+  if (count)
+    check_hip(hipMemcpy(lm.data(), lmd.p, sizeof(ps_local_max) * (size_t)count, hipMemcpyDeviceToHost), "hipMemcpy");
+  const ps_exp_param ep = to_exp_param(exp_param);
+  for (const ps_local_max &h : lm) {
+    // scale_from_index (partapp_aux.hpp:44-55) on the float proto fields
+    const double mn = ep.min_object_scale, mx = ep.max_object_scale;
+    const double scale = mn == mx ? mn : mn + (mx - mn) / (double)ep.num_scale_steps * (0.5 + h.d);
+    auto *hyp = hypothesis_list.add_hyp();
+    hyp->set_scale((float)scale);
+    hyp->set_x((float)h.x);
+    hyp->set_y((float)h.y);
+    hyp->set_score(h.score);
+  }
+}
+
+// The object hypotheses findObjectImageRotJoints derives from the root
+// posterior (objectdetect_findrot.cpp:1037-1048): up to 1000 local maxima,
+// each tagged with flip and moved from the root position to the bounding
+// box by the window's bbox offset (truncated to int).  PartAppT also needs
+// m_window_param.bbox_offset_x() / bbox_offset_y(); HypListT also needs
+// hyp_size(), hyp(i).x() / y() and mutable_hyp(i)->set_flip / set_x / set_y.
+template <class PartAppT, class Grid3, class HypListT>
+void rootHypotheses(const PartAppT &part_app, const Grid3 &root_part_posterior, bool flip,
+                    HypListT &hypothesis_list) {
+  const int max_hypothesis_number = 1000;
+  findLocalMax(part_app.m_exp_param, root_part_posterior, hypothesis_list, max_hypothesis_number);
+  for (int i = 0; i < hypothesis_list.hyp_size(); ++i) {
+    hypothesis_list.mutable_hyp(i)->set_flip(flip);
+    const int bbox_x = (int)(hypothesis_list.hyp(i).x() + part_app.m_window_param.bbox_offset_x());
+    const int bbox_y = (int)(hypothesis_list.hyp(i).y() + part_app.m_window_param.bbox_offset_y());
+    hypothesis_list.mutable_hyp(i)->set_x(bbox_x);
+    hypothesis_list.mutable_hyp(i)->set_y(bbox_y);
+  }
+}
+
+// best_conf of findObjectImageRotJoints (objectdetect_findrot.cpp:1005-1008):
+// [nParts][7], row p = best_part_hyp[p][0].toVect() = (scaleidx, scale,
+// rotidx, rot, x, y, score) (objectdetect.h:139-159).
+template <class PartHypT>
+std::vector<float> bestConf(const std::vector<std::vector<PartHypT>> &best_part_hyp) {
+  std::vector<float> out;
+  out.reserve(best_part_hyp.size() * 7);
+  for (const auto &hyps : best_part_hyp) {
+    if (hyps.empty()) throw std::invalid_argument("best_part_hyp has a part without hypotheses");
+    const auto v = hyps[0].toVect();
+    for (int i = 0; i < 7; ++i) out.push_back((float)v[i]);
+  }
+  return out;
+}
+
 // MAP record of the convenience overload, which discards it.
 struct IgnoredHyp {
   template <class E>

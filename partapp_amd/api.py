@@ -328,6 +328,23 @@ def find_local_max(grids, max_n=1000, stream=None):
     return res[0] if single else res
 
 
+def ingest_scores(score_grid, Tig, interpolate, height, width, stream=None):
+    """PartApp::loadScoreGrid + clip_scores_fill + computeLogGrid
+    (partapp.cpp:866-899, objectdetect_findrot.cpp:834-845) on the device.
+
+    score_grid: torch CUDA float32 [n][gh][gw]; Tig: [n][3][3] (image = Tig *
+    grid).  Returns the log unaries, torch CUDA float32 [n][height][width]."""
+    import torch
+    n, gh, gw = score_grid.shape
+    T = np.ascontiguousarray(np.asarray(Tig, dtype=np.float64).reshape(n, 9))
+    out = torch.empty((n, height, width), dtype=torch.float32, device=score_grid.device)
+    src = score_grid.contiguous()
+    check(lib().ps_ingest_scores(src.data_ptr(), n, gh, gw, T.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 1 if interpolate else 0, int(height), int(width), out.data_ptr(),
+                                 _stream_handle(stream)), "ps_ingest_scores")
+    return out
+
+
 # ----------------------------------------------------------- synthetic data
 
 PERSON_ROOT = 4

@@ -192,6 +192,15 @@ void eigen_2x2(const double M[4], double V[4], double E[2]) {
 }
 }  // namespace
 
+void warp_matrix(const double T21[9], bool gather, double out[6]) {
+  Mat3 T;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) T.m[i][j] = T21[3 * i + j];
+  const Mat3 M = gather ? mat_mul(mat_inverse(T), mat_ident()) : mat_mul(mat_ident(), T);
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = M.m[i][j];
+}
+
 // The eigenbasis path of one message: multi_array_filter.hpp:334-369 with
 // offset 0, transform_grid_resize (multi_array_transform.hpp:284-305) and
 // get_transformed_bbox (homogeneous_coord.cpp:128-146).

@@ -64,6 +64,12 @@ int build_message_plan(const ps_exp_param &ep, int H, int W, const double offset
                        const double C[4], double rot_mean, double rot_sigma, double scale, bool is_sparse,
                        MessagePlan &out);
 
+// Rows 0-1 of the sampling matrix of a fixed-size warp with the homogeneous
+// 3x3 T21 (row major): T21 itself for TM_DIRECT (source -> output), else
+// inverse(T21) for TM_BILINEAR / TM_NEAREST (output -> source)
+// (multi_array_transform.hpp:131-136, 167-196).
+void warp_matrix(const double T21[9], bool gather, double out[6]);
+
 double rot_from_index(const ps_exp_param &ep, int rotidx);
 double scale_from_index(const ps_exp_param &ep, int scaleidx);
 int index_from_rot(const ps_exp_param &ep, double rot);

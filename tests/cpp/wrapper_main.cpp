@@ -3,7 +3,8 @@
 // PartHyp), the way partapp's findObjectImageRotJoints calls
 // computeRootPosteriorRot.  Reads the log unaries [P][S][R][H][W] and joints
 // from a binary file, writes the root posterior, the log marginals, the
-// best_part_hyp records and the masked unaries.
+// best_part_hyp records, the masked unaries, best_conf and the root
+// HypothesisList.
 //   wrapper_main <in.bin> <out.bin>
 #include <array>
 #include <cstdio>
@@ -35,9 +36,36 @@ struct PartConfig {
   int part_size() const { return (int)parts.size(); }
   const PartDef &part(int p) const { return parts[p]; }
 };
+struct WindowParam {
+  double bbox_offset_x() const { return -12.7; }
+  double bbox_offset_y() const { return 5.3; }
+};
 struct PartApp {
   ExpParam m_exp_param;
   PartConfig m_part_conf;
+  WindowParam m_window_param;
+};
+// protobuf-style HypothesisList (HypothesisList.proto)
+struct ObjectHypothesis {
+  float x_ = 0, y_ = 0, scale_ = 0, score_ = 0;
+  bool flip_ = false;
+  float x() const { return x_; }
+  float y() const { return y_; }
+  void set_x(float v) { x_ = v; }
+  void set_y(float v) { y_ = v; }
+  void set_scale(float v) { scale_ = v; }
+  void set_score(float v) { score_ = v; }
+  void set_flip(bool v) { flip_ = v; }
+};
+struct HypothesisList {
+  std::vector<ObjectHypothesis> h;
+  ObjectHypothesis *add_hyp() {
+    h.emplace_back();
+    return &h.back();
+  }
+  int hyp_size() const { return (int)h.size(); }
+  const ObjectHypothesis &hyp(int i) const { return h[i]; }
+  ObjectHypothesis *mutable_hyp(int i) { return &h[i]; }
 };
 struct Vec2 {
   double v[2];
@@ -67,7 +95,16 @@ struct Grid3 {
 struct PartHyp {
   int scaleidx, rotidx, x, y;
   float score;
-  PartHyp(const ExpParam &, int s, int r, int x_, int y_, float sc) : scaleidx(s), rotidx(r), x(x_), y(y_), score(sc) {}
+  double scale, rot;
+  PartHyp(const ExpParam &e, int s, int r, int x_, int y_, float sc) : scaleidx(s), rotidx(r), x(x_), y(y_), score(sc) {
+    scale = e.min_scale == e.max_scale ? (double)e.min_scale
+                                       : (double)e.min_scale + ((double)e.max_scale - e.min_scale) / e.S * (0.5 + s);
+    rot = e.min_rot == e.max_rot ? (double)e.min_rot
+                                 : (double)e.min_rot + ((double)e.max_rot - e.min_rot) / e.R * (0.5 + r);
+  }
+  std::array<float, 7> toVect() const {
+    return {(float)scaleidx, (float)scale, (float)rotidx, (float)rot, (float)x, (float)y, score};
+  }
 };
 
 template <class T>
@@ -111,8 +148,12 @@ int main(int argc, char **argv) {
   Grid3 root_post;
   std::vector<std::vector<PartHyp>> best;
   std::vector<std::vector<Grid3>> marg;
+  HypothesisList hyps;
+  std::vector<float> best_conf;
   try {
     partapp_ps::computeRootPosteriorRot(app, det, root_post, root, joints, false, true, 0, best, false, &marg);
+    partapp_ps::rootHypotheses(app, root_post, true, hyps);
+    best_conf = partapp_ps::bestConf(best);
   } catch (const std::exception &e) {
     fprintf(stderr, "%s\n", e.what());
     return 1;
@@ -133,6 +174,13 @@ int main(int argc, char **argv) {
   }
   for (int p = 0; p < P; ++p)
     for (int s = 0; s < S; ++s) fwrite(det[p][s].data(), sizeof(float), (size_t)R * H * W, o);
+  fwrite(best_conf.data(), sizeof(float), best_conf.size(), o);
+  const int nh = hyps.hyp_size();
+  fwrite(&nh, sizeof(int), 1, o);
+  for (const ObjectHypothesis &h : hyps.h) {
+    float rec[5] = {h.x_, h.y_, h.scale_, h.score_, h.flip_ ? 1.0f : 0.0f};
+    fwrite(rec, sizeof(float), 5, o);
+  }
   fclose(o);
   printf("ok\n");
   return 0;

@@ -1,8 +1,9 @@
 """The C++ interface (include/partapp_ps.hpp) called the way partapp's
 findObjectImageRotJoints calls computeRootPosteriorRot, with stand-ins of
 partapp's PartApp / Joint / FloatGrid3 / PartHyp types, checked against the
-oracle: marginals, root posterior, best_part_hyp of the last scale, and the
-in-place masking of the caller's score grids."""
+oracle: marginals, root posterior, best_part_hyp of the last scale, the
+in-place masking of the caller's score grids, best_conf and the root
+HypothesisList."""
 import os
 import subprocess
 import sys
@@ -65,6 +66,13 @@ def test_cpp_wrapper_matches_oracle(gpu, tmp_path):
             o += 20
         hyps.append(hp)
     masked = np.frombuffer(raw, np.float32, n_grid, o).reshape(P, S, R, H, W)
+    o += 4 * n_grid
+    best_conf = np.frombuffer(raw, np.float32, 7 * P, o).reshape(P, 7)
+    o += 28 * P
+    nh = int(np.frombuffer(raw, np.int32, 1, o)[0])
+    o += 4
+    root_hyps = np.frombuffer(raw, np.float32, 5 * nh, o).reshape(nh, 5)
+    assert o + 20 * nh == len(raw)
 
     r32, p32, b32, s32 = oracle.root_posterior_rot(oracle.exp_param_from(ep), model.is_detect, model.is_upright, 4,
                                                    joints, u)
@@ -80,6 +88,22 @@ def test_cpp_wrapper_matches_oracle(gpu, tmp_path):
         lm = oracle.find_local_max(marg[p, S - 1], 1000)
         assert [h[0] for h in hyps[p][1:]] == [(0, int(d), int(y), int(x)) for d, x, y, v in lm]
         assert len(hyps[p]) - 1 == len(lm) > 0
+    # best_conf rows = toVect() of best_part_hyp[p][0] (objectdetect_findrot.cpp:1005-1008)
+    for p in range(P):
+        (scaleidx, rotidx, y, x), score = hyps[p][0]
+        scale = 0.7 + (1.3 - 0.7) / S * (0.5 + scaleidx)
+        rot = -180.0 + 360.0 / R * (0.5 + rotidx)
+        np.testing.assert_allclose(best_conf[p], [scaleidx, scale, rotidx, rot, x, y, score], rtol=1e-6)
+    # root HypothesisList (:1037-1048): local maxima of the root posterior over
+    # (scale, y, x), flip set, moved by the bbox offset (-12.7, 5.3) truncated to int
+    lm = oracle.find_local_max(root, 1000)
+    assert nh == len(lm) > 0
+    for (d, x, y, v), h in zip(lm, root_hyps):
+        # scale_from_index on the float proto fields (partapp_aux.hpp:44-55)
+        mn, mx = float(np.float32(0.7)), float(np.float32(1.3))
+        scale = np.float32(mn + (mx - mn) / S * (0.5 + d))
+        assert (h[0], h[1]) == (int(x - 12.7), int(y + 5.3))
+        assert h[2] == scale and h[3] == np.float32(v) and h[4] == 1.0
     finite = r32 > -1e5
     np.testing.assert_allclose(root[finite], r32[finite], rtol=2e-5, atol=2e-5)
     # masks written into the caller's grids as the reference does (:509-551)

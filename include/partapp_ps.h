@@ -124,6 +124,24 @@ int ps_plan_execute(ps_plan *plan, const float *log_unary, float *log_marginal, 
 int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, float *log_marginal, float *root_posterior,
                           ps_map *map_dev, void *stream);
 
+/* Arithmetic of the spatial Gaussian passes (the rest of every message stays
+ * FP32).  PS_METHOD_FP32 (default) is the drop-in: sequential FP32 dot
+ * products in the reference's order.  The BF16 methods run both passes as
+ * banded-Toeplitz GEMMs on the matrix cores with FP32 accumulation and a
+ * power-of-two rescale per rotation slice: PS_METHOD_BF16 rounds data and
+ * taps to bf16, PS_METHOD_BF16X3 splits both into bf16 hi + lo and adds three
+ * products (about 16 significant bits).  Available for 24 rotations and half
+ * widths up to 111; a level of messages outside that runs FP32
+ * (ps_plan_method_levels counts the levels that use the method).  No
+ * reference counterpart: a precision option of this library. */
+enum {
+  PS_METHOD_FP32 = 0,
+  PS_METHOD_BF16 = 1,
+  PS_METHOD_BF16X3 = 3
+};
+int ps_plan_set_method(ps_plan *plan, int method);
+int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
+
 /* Per-kernel timing of a plan's executions, for measurement: while enabled,
  * every launch of ps_plan_execute is bracketed by HIP events on its stream.
  * ps_plan_read_profile waits for the recorded launches and returns, per kernel

@@ -212,6 +212,17 @@ class Plan:
 
     KERNEL_CLASSES = ("reset", "leaf_max", "rowpass", "colpass", "root", "map_decode")
 
+    METHODS = {"fp32": 0, "bf16": 1, "bf16x3": 3}
+
+    def set_method(self, method):
+        """Arithmetic of the spatial passes: 'fp32' (the drop-in), 'bf16' or
+        'bf16x3' (banded-Toeplitz GEMMs on the matrix cores).  Returns
+        (levels, levels that use the method)."""
+        check(lib().ps_plan_set_method(self.handle, self.METHODS[method]), "ps_plan_set_method")
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().ps_plan_method_levels(self.handle, ctypes.byref(a), ctypes.byref(b)), "ps_plan_method_levels")
+        return a.value, b.value
+
     def set_profiling(self, enable=True):
         check(lib().ps_plan_set_profiling(self.handle, int(bool(enable))), "ps_plan_set_profiling")
 

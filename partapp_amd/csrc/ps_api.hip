@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -31,6 +32,14 @@ hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
+size_t rowpass_mfma_lds_bytes(int R, int W, int KK, int terms);
+size_t colpass_mfma_lds_bytes(int KK, int terms);
+hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes);
+hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
+                               const uint4 *frags, int R, int terms, int max_h, int max_w, size_t lds_bytes,
+                               hipStream_t s);
+hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
+                               int R, int terms, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
 }  // namespace ps
 
@@ -50,7 +59,26 @@ struct Level {
   size_t lds = 0;      // row pass
   size_t col_lds = 0;  // column pass
   long long od_cells = 0, od_hw = 0;  // max R*th*tw and H*W of the off-diagonal jobs
+  bool mfma_ok = false;                // every diagonal job fits the BF16 Toeplitz passes
+  size_t mfma_row_lds[2] = {0, 0};     // 1 and 3 terms
+  size_t mfma_col_lds[2] = {0, 0};
 };
+
+// Host bf16 of a float, round to nearest even (finite inputs).
+uint16_t bf16_rne(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float bf16_float(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// k-steps of 16 of a 32-wide Toeplitz tile with half width k
+int mfma_kk(int k) { return (32 + 2 * k + 15) / 16; }
 
 int hip_error(hipError_t e, const char *what) {
   return set_error(PS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -93,6 +121,8 @@ struct ps_plan {
   MapJob *maps = nullptr;
   float *taps = nullptr;
   int *shifts = nullptr;
+  uint4 *frags = nullptr;
+  int terms = 0;  // spatial filter: 0 FP32, 1 or 3 BF16 Toeplitz terms
   float *scratch = nullptr;
   size_t scratch_elems = 0;
   unsigned *keys = nullptr;
@@ -165,7 +195,29 @@ struct Builder {
   std::vector<MapJob> maps;
   std::vector<float> taps;
   std::vector<int> shifts;
+  std::vector<uint16_t> frags;  // bf16 Toeplitz fragments, 8 per 16-byte unit
   explicit Builder(const Model &mm) : m(mm) {}
+
+  // Fragment tables of one filter for the BF16 passes: lane l at step kk holds
+  // f[16 kk + 8 (l / 32) + t - l % 32], t = 0..7 (zero off the band); the
+  // bf16 hi parts, then the lo parts (f - hi).  Returns the offset in 16-byte units.
+  int add_frags(const std::vector<float> &f, int KK) {
+    const int off = (int)(frags.size() / 8);
+    const int L = (int)f.size();
+    std::vector<uint16_t> lo;
+    lo.reserve((size_t)KK * 64 * 8);
+    for (int kk = 0; kk < KK; ++kk)
+      for (int l = 0; l < 64; ++l)
+        for (int t = 0; t < 8; ++t) {
+          const int i = 16 * kk + 8 * (l / 32) + t - l % 32;
+          const float v = (i >= 0 && i < L) ? f[i] : 0.0f;
+          const uint16_t hi = bf16_rne(v);
+          frags.push_back(hi);
+          lo.push_back(bf16_rne(v - bf16_float(hi)));
+        }
+    frags.insert(frags.end(), lo.begin(), lo.end());
+    return off;
+  }
 
   int add_taps(const std::vector<float> &f, int align) {
     while (taps.size() % 16) taps.push_back(0.0f);
@@ -203,6 +255,14 @@ struct Builder {
       J.od = 0;
       J.taps_x = add_taps(mp.fx, 16);
       J.taps_y = add_taps(mp.fy, 16);
+      J.kk_x = mfma_kk(mp.kx);
+      J.kk_y = mfma_kk(mp.ky);
+      if (J.kk_x <= kMfmaMaxKK && J.kk_y <= kMfmaMaxKK) {
+        J.frag_x = add_frags(mp.fx, J.kk_x);
+        J.frag_y = add_frags(mp.fy, J.kk_y);
+      } else {
+        J.kk_x = J.kk_y = 0;  // too wide for the BF16 passes: FP32 only
+      }
     }
     J.shifts = (int)shifts.size();
     shifts.insert(shifts.end(), mp.fwd_ox.begin(), mp.fwd_ox.end());
@@ -459,13 +519,17 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->items = items;
   pl->max_rows = max_rows;
   pl->max_hw = max_hw;
+  // the slice max keys of the BF16 passes follow the per-item keys
+  const int slice_key_base = items * 2 * P;
   auto add_levels = [&](std::vector<std::vector<MsgJob>> &lv, std::vector<Level> &dst) -> int {
     for (auto &v : lv) {
       std::stable_partition(v.begin(), v.end(), [](const MsgJob &j) { return j.od == 0; });
       Level L;
       L.off = (int)B.jobs.size();
       L.count = (int)v.size();
-      for (const MsgJob &j : v) {
+      L.mfma_ok = R == 24;
+      for (MsgJob &j : v) {
+        j.slice_key = slice_key_base + (int)(B.jobs.size() + (&j - v.data())) * R;
         L.max_h = std::max(L.max_h, j.H);
         L.max_w = std::max(L.max_w, j.W);
         L.lds = std::max(L.lds, rowpass_lds_bytes(R, j.W, j.kx));
@@ -475,8 +539,14 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         } else {
           ++L.n_diag;
           L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
+          if (j.kk_x < 1 || j.kk_y < 1) L.mfma_ok = false;
+          for (int t = 0; t < 2 && L.mfma_ok; ++t) {
+            L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kk_x, t ? 3 : 1));
+            L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t ? 3 : 1));
+          }
         }
       }
+      if (L.mfma_row_lds[1] > kLdsLimit || L.mfma_col_lds[1] > kLdsLimit) L.mfma_ok = false;
       if (L.lds > kLdsLimit || L.col_lds > kLdsLimit)
         return set_error(PS_ERR_UNSUPPORTED, "row pass needs " + std::to_string(L.lds) +
                                                  " bytes of LDS (R * width too large for one workgroup)");
@@ -493,7 +563,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   }
   pl->n_roots = (int)B.roots.size();
   pl->n_maxes = (int)B.maxes.size();
-  pl->n_keys = items * 2 * P;
+  pl->n_keys = slice_key_base + (int)B.jobs.size() * R;
   pl->n_map = items * P;
   pl->scratch_elems = scratch_off;
 
@@ -505,7 +575,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   size_t o_maps = align(o_maxes + B.maxes.size() * sizeof(MaxJob));
   size_t o_taps = align(o_maps + B.maps.size() * sizeof(MapJob));
   size_t o_shifts = align(o_taps + B.taps.size() * sizeof(float));
-  size_t o_keys = align(o_shifts + B.shifts.size() * sizeof(int));
+  size_t o_frags = align(o_shifts + B.shifts.size() * sizeof(int));
+  size_t o_keys = align(o_frags + B.frags.size() * sizeof(uint16_t));
   size_t o_mkeys = align(o_keys + (size_t)pl->n_keys * sizeof(unsigned));
   size_t total = align(o_mkeys + (size_t)pl->n_map * sizeof(unsigned long long));
   hipError_t e = hipMalloc(&pl->tables, total);
@@ -520,6 +591,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->maps = reinterpret_cast<MapJob *>(base + o_maps);
   pl->taps = reinterpret_cast<float *>(base + o_taps);
   pl->shifts = reinterpret_cast<int *>(base + o_shifts);
+  pl->frags = reinterpret_cast<uint4 *>(base + o_frags);
   pl->keys = reinterpret_cast<unsigned *>(base + o_keys);
   pl->map_keys = reinterpret_cast<unsigned long long *>(base + o_mkeys);
   std::vector<char> host(total, 0);
@@ -529,6 +601,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   std::memcpy(host.data() + o_maps, B.maps.data(), B.maps.size() * sizeof(MapJob));
   std::memcpy(host.data() + o_taps, B.taps.data(), B.taps.size() * sizeof(float));
   std::memcpy(host.data() + o_shifts, B.shifts.data(), B.shifts.size() * sizeof(int));
+  std::memcpy(host.data() + o_frags, B.frags.data(), B.frags.size() * sizeof(uint16_t));
   e = hipMemcpy(pl->tables, host.data(), total, hipMemcpyHostToDevice);
   if (e == hipSuccess && pl->scratch_elems) {
     e = hipMalloc(&pl->scratch, pl->scratch_elems * sizeof(float));
@@ -547,11 +620,46 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   for (auto &L : pl->up) max_lds = std::max(max_lds, L.lds);
   for (auto &L : pl->down) max_lds = std::max(max_lds, L.lds);
   e = rowpass_set_lds_limit(max_lds);
+  size_t mrow = 0, mcol = 0;
+  for (auto *lv : {&pl->up, &pl->down})
+    for (auto &L : *lv)
+      if (L.mfma_ok) {
+        mrow = std::max(mrow, L.mfma_row_lds[1]);
+        mcol = std::max(mcol, L.mfma_col_lds[1]);
+      }
+  if (e == hipSuccess) e = mfma_set_lds_limit(mrow, mcol);
   if (e != hipSuccess) {
     ps_plan_destroy(pl);
     return hip_error(e, "hipFuncSetAttribute");
   }
   *out = pl;
+  return PS_OK;
+}
+
+extern "C" int ps_plan_set_method(ps_plan *plan, int method) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  int terms;
+  switch (method) {
+    case PS_METHOD_FP32: terms = 0; break;
+    case PS_METHOD_BF16: terms = 1; break;
+    case PS_METHOD_BF16X3: terms = 3; break;
+    default: return set_error(PS_ERR_INVALID_ARG, "unknown method " + std::to_string(method));
+  }
+  if (terms != plan->terms && plan->gexec) {
+    (void)hipGraphExecDestroy(plan->gexec);
+    plan->gexec = nullptr;
+  }
+  plan->terms = terms;
+  return PS_OK;
+}
+
+extern "C" int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels) {
+  if (!plan || !levels || !mfma_levels) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  *levels = (int)(plan->up.size() + plan->down.size());
+  int n = 0;
+  for (auto *lv : {&plan->up, &plan->down})
+    for (auto &L : *lv) n += L.mfma_ok ? 1 : 0;
+  *mfma_levels = n;
   return PS_OK;
 }
 
@@ -612,13 +720,28 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     profile_end(pp, ev, s);
   }
   auto level = [&](const Level &L) {
+    const bool mf = pl->terms != 0 && L.mfma_ok;
+    const int ti = pl->terms == 3 ? 1 : 0;
+    const MsgJob *jobs = pl->jobs + L.off;
     long ev2 = profile_begin(pp, 2, s);
-    e = launch_rowpass(b, pl->jobs + L.off, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
+    if (mf) {
+      e = launch_rowpass_mfma(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms, L.max_h, L.max_w,
+                              L.mfma_row_lds[ti], s);
+      if (e == hipSuccess)
+        e = launch_rowpass(b, jobs + L.n_diag, L.count - L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w,
+                           L.lds, s);
+    } else {
+      e = launch_rowpass(b, jobs, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
+    }
     profile_end(pp, ev2, s);
     if (e != hipSuccess) return;
     // the column pass class also times the eigenbasis stages
     ev2 = profile_begin(pp, 3, s);
-    e = launch_colpass(b, pl->jobs + L.off, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
+    if (mf)
+      e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->terms, L.max_w,
+                              L.mfma_col_lds[ti], s);
+    else
+      e = launch_colpass(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
     if (e == hipSuccess)
       e = launch_offdiag(b, pl->jobs + L.off + L.n_diag, L.count - L.n_diag, pl->taps, pl->shifts, pl->R, L.od_cells,
                          L.od_hw, s);

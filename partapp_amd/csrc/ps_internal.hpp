@@ -12,6 +12,7 @@ constexpr int kMaxRot = 64;        // rotation masks are 64-bit words
 constexpr int kMaxRootChildren = 16;
 constexpr int kMaxTaps = 1000;     // multi_array_filter.hpp:243-244
 constexpr int kNoSourceRow = -(1 << 30);  // row offset of an output slice with no source slice
+constexpr int kMfmaMaxKK = 16;     // BF16 Toeplitz passes: half widths up to 111 (2k + 32 <= 16 * 16)
 
 // Which base pointer an offset is relative to.
 enum BufId : int { kBufUnary = 0, kBufMarginal = 1, kBufScratch = 2, kBufRoot = 3 };
@@ -74,6 +75,13 @@ struct MsgJob {
   GridRef dst2;       // Down with a grandchild: unary + message (next source), else buf -1
   int dst2_key;
   int map_slot;       // Down: argmax key of the child marginal
+  // BF16 Toeplitz variant of the x / y passes: k-steps of 16 and offsets (in
+  // 16-byte units) of the tap fragment tables (hi block, then lo block), and
+  // the first of R max keys of the row-filtered slices
+  int kk_x, kk_y;
+  int frag_x, frag_y;
+  int slice_key;
+  int pad2_;
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the

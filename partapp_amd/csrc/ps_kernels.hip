@@ -25,6 +25,7 @@
 #include <cstdint>
 
 #include "ps_device.hpp"
+#include "ps_rowgather.hpp"
 #include "ps_internal.hpp"
 #include "../../include/partapp_ps.h"
 
@@ -149,38 +150,6 @@ __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x
     }
   }
 }
-
-// Circular rotation filter with LR taps on a register column
-// (objectdetect_findrot.cpp:377-420, multi_array_filter.hpp:115-155):
-// out[r] = sum_j in[(r - n + j) mod R] f[j], taps in ascending order.
-template <int R, int LR>
-__device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], float (&o)[R], const float *fr) {
-  constexpr int n = (LR - 1) / 2;
-  float f[LR];
-#pragma unroll
-  for (int j = 0; j < LR; ++j) f[j] = fr[j];
-#pragma unroll
-  for (int ro = 0; ro < R; ++ro) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int j = 0; j < LR; ++j) acc = fmaf(e[((ro - n + j) % R + R) % R], f[j], acc);
-    o[ro] = acc;
-  }
-}
-template <int R, int LR>
-struct RotDispatch {
-  static __device__ __forceinline__ void run(int lr, const float (&e)[R], float (&o)[R], const float *fr) {
-    if (lr == LR) rot_filter_fixed<R, LR>(e, o, fr);
-    else RotDispatch<R, LR - 2>::run(lr, e, o, fr);
-  }
-};
-template <int R>
-struct RotDispatch<R, -1> {
-  static __device__ __forceinline__ void run(int, const float (&)[R], float (&o)[R], const float *) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) o[r] = 0.0f;
-  }
-};
 
 // Row pass for a rotation count fixed at compile time: each lane owns one
 // column x; its R source values are loaded together (clamped addresses, then

@@ -1,0 +1,135 @@
+"""The BF16 banded-Toeplitz spatial passes (PS_METHOD_BF16 / _BF16X3) against
+the FP64 oracle: per-part relative L1 of the normalised marginals and MAP
+agreement, on the same seeded inputs as the FP32 parity tests.  The FP32
+method stays the drop-in; these methods carry their own error budget
+(<= 1e-3 relative L1 against FP64, MAP identical)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from partapp_amd.api import ExpParam, Plan, config_exp_param, decode_maps, person_model, synth_joints, synth_unary
+
+from test_gpu_offdiag import full_cov_joints
+from test_gpu_parity import oracle_run
+
+pytestmark = pytest.mark.gpu
+
+BUDGET_L1 = 1e-3
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_method(model, unaries, joints, method, graph=False):
+    import torch
+    sizes = [u.shape[3:] for u in unaries]
+    plan = Plan(model, sizes, joints)
+    levels, used = plan.set_method(method)
+    dev = torch.device("cuda:0")
+    un = torch.from_numpy(np.concatenate([u.ravel() for u in unaries])).to(dev)
+    marg = torch.full_like(un, float("nan"))
+    root = torch.full((plan.root_numel,), float("nan"), device=dev)
+    maps = torch.zeros((plan.map_numel * 4,), dtype=torch.int32, device=dev)
+    if graph:
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            plan.execute(un, marg, root, maps, stream=s, graph=True)
+        s.synchronize()
+    else:
+        plan.execute(un, marg, root, maps)
+        torch.cuda.synchronize()
+    m = decode_maps(maps, plan.n_instances, plan.S, plan.P)
+    out = [(plan.instance_view(marg, i).cpu().numpy(), plan.instance_view(root, i, "root").cpu().numpy(), m[i])
+           for i in range(plan.n_instances)]
+    plan.close()
+    return out, (levels, used)
+
+
+def errors_vs_f64(model, u, joints, out):
+    marg, root, maps = out
+    P, S = u.shape[0], u.shape[1]
+    r64, p64, b64, s64 = oracle_run(model, u, joints, np.float64)
+    worst, map_bad = 0.0, []
+    for s in range(S):
+        for p in range(P):
+            worst = max(worst, oracle.rel_l1(oracle.normalized(marg[p, s]), oracle.normalized(p64[p, s])))
+            got = (int(maps[s, p]["rotidx"]), int(maps[s, p]["y"]), int(maps[s, p]["x"]))
+            if got != tuple(b64[s, p]):
+                map_bad.append((s, p, got, tuple(b64[s, p]), oracle.top2_gap(p64[p, s])))
+    return worst, map_bad
+
+
+@pytest.mark.parametrize("method", ["bf16x3", "bf16"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_bf16_sweep_error_vs_f64(gpu, method, seed):
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(seed)
+    model = person_model(ep, joints)
+    u = synth_unary(seed, 10, 1, 24, 112, 128)
+    say("bf16 sweep", method, "seed", seed)
+    outs, (levels, used) = run_method(model, [u], [joints], method)
+    assert used == levels == 4  # two upward and two downward levels
+    worst, map_bad = errors_vs_f64(model, u, joints, outs[0])
+    say("  %s seed %d: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (method, seed, worst, map_bad))
+    assert worst <= BUDGET_L1
+    assert not map_bad
+
+
+def test_bf16x3_cfg0_full_size(gpu):
+    ep = config_exp_param(0)
+    joints = synth_joints(0)
+    model = person_model(ep, joints)
+    u = synth_unary(0, 10, 1, 24, 200, 200)
+    outs, _ = run_method(model, [u], [joints], "bf16x3")
+    worst, map_bad = errors_vs_f64(model, u, joints, outs[0])
+    say("bf16x3 cfg0: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (worst, map_bad))
+    assert worst <= BUDGET_L1 and not map_bad
+
+
+def test_bf16x3_uneven_scales_masks_and_offdiagonal(gpu):
+    # partial tiles (W not a multiple of 32, H not of 64), three scales, masks,
+    # and an instance whose off-diagonal joints keep the FP32 eigenbasis path
+    # inside the same levels
+    ep = ExpParam(-180.0, 180.0, 24, 0.7, 1.3, 3, 0.1)
+    is_upright = [0] * 10
+    is_upright[4] = is_upright[5] = 1
+    joints = [synth_joints(60), full_cov_joints(61, every=2)]
+    model = person_model(ep, joints[0], is_upright=is_upright)
+    sizes = [(152, 176), (136, 144)]
+    us = [synth_unary(70 + i, 10, 3, 24, h, w) for i, (h, w) in enumerate(sizes)]
+    outs, (levels, used) = run_method(model, us, joints, "bf16x3")
+    assert used == levels
+    for i in range(2):
+        worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
+        say("bf16x3 uneven %d: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (i, worst, map_bad))
+        assert worst <= BUDGET_L1 and not map_bad
+
+
+def test_bf16_graph_equals_eager(gpu):
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(3)
+    model = person_model(ep, joints)
+    u = synth_unary(3, 10, 1, 24, 96, 104)
+    eager, _ = run_method(model, [u], [joints], "bf16x3")
+    graph, _ = run_method(model, [u], [joints], "bf16x3", graph=True)
+    np.testing.assert_array_equal(eager[0][0], graph[0][0])
+    np.testing.assert_array_equal(eager[0][1], graph[0][1])
+
+
+def test_method_fallback_for_wide_filters(gpu):
+    # a joint with sigma 40 px (half width 120) does not fit the BF16 passes:
+    # its levels run FP32 and the result still matches the FP32 oracle
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(5)
+    from partapp_amd.api import Joint
+    j = joints[0]
+    joints[0] = Joint(j.child_idx, j.parent_idx, j.offset_c, j.offset_p, (1600.0, 0, 0, 25.0), j.rot_mean, j.rot_sigma)
+    model = person_model(ep, joints)
+    u = synth_unary(5, 10, 1, 24, 112, 128)
+    outs, (levels, used) = run_method(model, [u], [joints], "bf16x3")
+    assert used < levels
+    worst, map_bad = errors_vs_f64(model, u, joints, outs[0])
+    assert worst <= BUDGET_L1 and not map_bad

@@ -6,13 +6,22 @@ per-image spatial model (seeds 0..31).  One step = the whole sweep of the
 batch (upward pass, root, downward pass, marginals of all parts, MAP, root
 posterior) with the unaries already resident in HBM.
 
+The metric compares the chosen method (the spatial Gaussian passes as BF16
+banded-Toeplitz GEMMs on the matrix cores: the x pass bf16 x bf16, the y pass
+with data and taps split into bf16 hi + lo) with the plain FP32 sweep (the
+drop-in) on the same plan and inputs, in the same run;
+`value` is the chosen method's images/s.  Its error against the FP64 oracle
+is measured live on a sample of the instances (relative L1 of the normalised
+marginals per part, MAP agreement).
+
 Prints one JSON line.  `roofline` is measured live over the timed steps with
-HIP events around each launch (on the stream the kernels run on); the
-dominant kernel's algorithmic FP32 flops per launch (2 x states x taps of the
-filters it evaluates, per message) over its average launch time, against the
-published FP32 peak.  `cpu_baseline` times the oracle (the float32 CPU
-restatement of the reference) on a bounded sample of the same instances,
-one instance per thread.
+HIP events around each launch (on the stream the kernels run on): the
+dominant kernel's algorithmic HBM bytes per launch (the grids it must read
+and write) over its average launch time, against the published HBM peak;
+the issued MFMA flops of the Toeplitz GEMMs over the BF16 dense peak are
+reported beside it.  `cpu_baseline` times the oracle (the float32 CPU
+restatement of the reference) on a bounded sample of the same instances, one
+instance per thread.
 """
 import argparse
 import concurrent.futures as cf
@@ -28,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "images/sec of 10-part marginals @300x300x24rot; chosen vs plain FP32 GPU at <=1e-3 L1"
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -38,21 +48,23 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--size", type=int, default=300)
+    ap.add_argument("--method", default="bf16mix", choices=["bf16mix", "bf16x3", "bf16x2", "bf16", "fp32"],
+                    help="the chosen method (value); plain FP32 is timed beside it")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison")
+    ap.add_argument("--check", type=int, default=2, help="instances checked against the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel roofline)")
     return ap.parse_args()
 
 
-def workload_flops(plan_sizes, joints, ep):
-    """Algorithmic FP32 flops per kernel class, per sweep: 2 * states * taps.
-    Row pass: rotation + x taps; column pass: y taps (one message each)."""
+def message_plans(plan_sizes, joints, ep):
+    """(H, W, rot taps, kx, ky) of every message of the sweep, both directions."""
     from partapp_amd import _lib
     L = _lib.lib()
     R = ep.num_rotation_steps
-    row = col = 0.0
+    out = []
     for (H, W), js in zip(plan_sizes, joints):
-        n = R * H * W
         for s in range(ep.num_scale_steps):
             scale = ep.scale_from_index(s)
             for j in js:
@@ -65,9 +77,53 @@ def workload_flops(plan_sizes, joints, ep):
                                                  ints[3], ctypes.byref(rl), None, ctypes.byref(kx),
                                                  ctypes.byref(ky))
                     assert st == 0
-                    row += 2.0 * n * (rl.value + 2 * kx.value + 1)
-                    col += 2.0 * n * (2 * ky.value + 1)
+                    out.append((H, W, rl.value, kx.value, ky.value))
+    return out
+
+
+def workload_flops(msgs, R):
+    """Algorithmic FP32 flops per kernel class, per sweep: 2 * states * taps.
+    Row pass: rotation + x taps; column pass: y taps (one message each)."""
+    row = sum(2.0 * R * H * W * (rl + 2 * kx + 1) for H, W, rl, kx, ky in msgs)
+    col = sum(2.0 * R * H * W * (2 * ky + 1) for H, W, rl, kx, ky in msgs)
     return row, col
+
+
+def mfma_flops(msgs, R, row_terms, col_terms):
+    """Issued MFMA flops of the Toeplitz GEMMs per sweep (32x32x16 tiles, K
+    rounded up to steps of 16, R = 24 rotation rows padded to 32)."""
+    tile = 2.0 * 32 * 32 * 16
+    fl = 0.0
+    for H, W, rl, kx, ky in msgs:
+        kkx, kky = (32 + 2 * kx + 15) // 16, (32 + 2 * ky + 15) // 16
+        fl += tile * row_terms * kkx * ((W + 31) // 32) * H  # row pass: one 32-row tile per output row
+        fl += tile * col_terms * kky * 4 * R * ((W + 63) // 64) * ((H + 63) // 64)  # column pass: 4 waves / 64x64
+    return fl
+
+
+def grid_traffic(model):
+    """Grids (R*H*W floats) each kernel class must read or write per sweep and
+    instance, given the row pass / column pass split: row pass = source read
+    + row-filtered grid write; column pass = row-filtered grid read, the
+    destination write(s) and the unary / stored belief its epilogue adds."""
+    P, root = model.num_parts, model.root
+    parent = {j.child_idx: j.parent_idx for j in model.joints}
+    children = {p: [c for c, q in parent.items() if q == p] for p in range(P)}
+    row = col = 0
+    for c, p in parent.items():
+        leaf = not children[c]
+        # upward message c -> p
+        row += (0 if (leaf and not model.is_detect[c]) else 1) + 1
+        col += 1 + 1 + (1 if (p != root and model.is_detect[p]) else 0)
+        # downward message p -> c
+        row += 2
+        # (grid read, marginal write, unary read; with a grandchild also the
+        # stored belief read and the next source write)
+        col += 1 + 1 + 1 + (2 if children[c] else 0)
+    k = len(children[root])
+    root_grids = k + 1 + 1 + k  # messages + unary read, marginal + log_from_root writes
+    leaf_grids = sum(1 for c in range(P) if c != root and not children[c] and model.is_detect[c])
+    return {"rowpass": row, "colpass": col, "root": root_grids, "leaf_max": leaf_grids}
 
 
 def cpu_baseline_timed(ep, sizes, joints, seeds, budget_s):
@@ -96,6 +152,63 @@ def cpu_baseline_timed(ep, sizes, joints, seeds, budget_s):
     return images, busy, cores
 
 
+def time_method(plan, method, unary, marg, root, maps, stream, args):
+    """Warmup, then EXACTLY args.steps timed steps; returns (ms, profile)."""
+    import torch
+    plan.set_method(method)
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            plan.execute(unary, marg, root, maps, stream=stream, graph=args.graph)
+        stream.synchronize()
+        if not args.graph:
+            plan.set_profiling(True)
+            plan.read_profile()  # clear
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for _ in range(args.steps):
+            plan.execute(unary, marg, root, maps, stream=stream, graph=args.graph)
+        t1.record(stream)
+        stream.synchronize()
+        torch.cuda.synchronize()
+    prof = None
+    if not args.graph:
+        prof = plan.read_profile()
+        plan.set_profiling(False)
+    return t0.elapsed_time(t1), prof
+
+
+def check_vs_f64(plan, model, marg, maps, ep, joints, seeds, sizes, k):
+    """Relative L1 (normalised marginals, per part) and MAP agreement of the
+    last executed sweep against the FP64 oracle on the first k instances."""
+    import numpy as np
+
+    import oracle
+    from partapp_amd.api import decode_maps, synth_unary
+    oep = oracle.exp_param_from(ep)
+    m = decode_maps(maps, plan.n_instances, plan.S, plan.P)
+    cores = min(os.cpu_count() or 1, 16)
+
+    def one(i):
+        H, W = sizes[i]
+        u = synth_unary(seeds[i], 10, 1, ep.num_rotation_steps, H, W, threads=1)
+        return oracle.root_posterior_rot(oep, model.is_detect, model.is_upright, model.root, joints[i],
+                                         u.astype(np.float64))
+
+    worst, agree, total = 0.0, 0, 0
+    with cf.ThreadPoolExecutor(max_workers=min(k, cores)) as ex:
+        refs = list(ex.map(one, range(k)))
+    for i, (r64, p64, b64, s64) in enumerate(refs):
+        mg = plan.instance_view(marg, i).cpu().numpy()
+        for p in range(plan.P):
+            worst = max(worst, oracle.rel_l1(oracle.normalized(mg[p, 0]), oracle.normalized(p64[p, 0])))
+            got = (int(m[i, 0, p]["rotidx"]), int(m[i, 0, p]["y"]), int(m[i, 0, p]["x"]))
+            agree += int(got == tuple(b64[0, p]))
+            total += 1
+    return worst, agree, total
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -111,17 +224,18 @@ def main():
     n, H = args.instances, args.size
     W = H
     ep = config_exp_param(1)
+    R = ep.num_rotation_steps
     seeds = list(range(n))
     joints = [synth_joints(s) for s in seeds]
     sizes = [(H, W)] * n
     model = person_model(ep, joints[0])
     plan = Plan(model, sizes, joints)
-    print("plan: %d instances %dx%dx%d, scratch %.2f GB" % (n, H, W, ep.num_rotation_steps,
-                                                          plan.scratch_bytes() / 1e9), file=sys.stderr, flush=True)
+    print("plan: %d instances %dx%dx%d, scratch %.2f GB" % (n, H, W, R, plan.scratch_bytes() / 1e9),
+          file=sys.stderr, flush=True)
     unary = torch.empty((plan.unary_numel,), dtype=torch.float32, device=dev)
     threads = min(os.cpu_count() or 1, 16)
     for i, s in enumerate(seeds):
-        u = synth_unary(s, 10, 1, ep.num_rotation_steps, H, W, threads=threads)
+        u = synth_unary(s, 10, 1, R, H, W, threads=threads)
         o = plan.unary_offsets[i]
         unary[o:o + u.size].copy_(torch.from_numpy(u.ravel()))
     marg = torch.empty_like(unary)
@@ -129,53 +243,59 @@ def main():
     maps = torch.zeros((plan.map_numel * 4,), dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
     print("inputs resident: %.2f GB" % (unary.numel() * 4 / 1e9), file=sys.stderr, flush=True)
+    stream = torch.cuda.Stream() if args.graph else torch.cuda.current_stream()
 
-    stream = torch.cuda.current_stream()
-    if args.graph:
-        stream = torch.cuda.Stream()
-    with torch.cuda.stream(stream):
-        for _ in range(args.warmup):
-            plan.execute(unary, marg, root, maps, stream=stream, graph=args.graph)
-        stream.synchronize()
-        print("warmup done", file=sys.stderr, flush=True)
-        if not args.graph:
-            plan.set_profiling(True)
-            plan.read_profile()  # clear
-        t0 = torch.cuda.Event(enable_timing=True)
-        t1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        wall0 = time.perf_counter()
-        t0.record(stream)
-        for _ in range(args.steps):
-            plan.execute(unary, marg, root, maps, stream=stream, graph=args.graph)
-        t1.record(stream)
-        stream.synchronize()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - wall0
-    ms = t0.elapsed_time(t1)
+    fp32 = None
+    if not args.no_fp32 and args.method != "fp32":
+        ms32, _ = time_method(plan, "fp32", unary, marg, root, maps, stream, args)
+        fp32 = {"value": round(n * args.steps / (ms32 / 1e3), 3), "ms_per_step": round(ms32 / args.steps, 4)}
+        print("plain fp32: %.3f ms/step" % (ms32 / args.steps), file=sys.stderr, flush=True)
+
+    ms, prof = time_method(plan, args.method, unary, marg, root, maps, stream, args)
     ms_per_step = ms / args.steps
     value = n * args.steps / (ms / 1e3)
-    print("timed %d steps: %.3f ms/step (wall %.3f s)" % (args.steps, ms_per_step, wall), file=sys.stderr,
-          flush=True)
+    print("%s: timed %d steps, %.3f ms/step" % (args.method, args.steps, ms_per_step), file=sys.stderr, flush=True)
 
+    err = None
+    if args.check > 0:
+        k = min(args.check, n)
+        worst, agree, total = check_vs_f64(plan, model, marg, maps, ep, joints, seeds, sizes, k)
+        err = {"max_rel_l1": float("%.3g" % worst), "map_agreement": "%d/%d" % (agree, total),
+               "sample": "%d of %d instances vs the FP64 oracle" % (k, n)}
+        print("error vs fp64: %s" % err, file=sys.stderr, flush=True)
+
+    msgs = message_plans(sizes, joints, ep)
+    grid_bytes = 4.0 * R * H * W
     roofline = None
-    if not args.graph:
-        prof = plan.read_profile()
-        plan.set_profiling(False)
-        row_f, col_f = workload_flops(sizes, joints, ep)
-        flops = {"rowpass": row_f, "colpass": col_f}
+    if prof is not None:
         for k, (t, c) in prof.items():
             print("  %-10s %9.3f ms total, %4d launches" % (k, t, c), file=sys.stderr)
-        dom = max(("rowpass", "colpass"), key=lambda k: prof[k][0])
+        dom = max(("rowpass", "colpass", "root"), key=lambda k: prof[k][0])
         t_dom, n_dom = prof[dom]
+        traffic_grids = grid_traffic(model)
+        bytes_per_step = traffic_grids[dom] * grid_bytes * n
         launches_per_step = n_dom / args.steps
-        flops_per_launch = flops[dom] / launches_per_step
-        avg_s = t_dom / n_dom / 1e3
-        achieved = flops_per_launch / avg_s / 1e12
-        roofline = {"bound": "fp32", "kernel": dom, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                    "avg_launch_ms": round(t_dom / n_dom, 4),
+        avg_ms = t_dom / n_dom
+        achieved = bytes_per_step / launches_per_step / (avg_ms / 1e3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_ms": round(avg_ms, 4),
                     "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()}}
+        # the whole sweep against the algorithmic minimum B_alg = 4 B * N_s * (2P + 4E)
+        b_alg = 4.0 * R * H * W * (2 * 10 + 4 * 9) * n
+        roofline["sweep_gbs"] = round(b_alg / (ms_per_step / 1e3) / 1e9, 1)
+        if args.method == "fp32":
+            row_f, col_f = workload_flops(msgs, R)
+            f = {"rowpass": row_f, "colpass": col_f}.get(dom)
+            if f:
+                roofline["fp32_tflops"] = round(f / (t_dom / args.steps / 1e3) / 1e12, 3)
+        else:
+            terms = {"bf16x3": (3, 3), "bf16x2": (2, 2), "bf16": (1, 1), "bf16mix": (1, 3)}[args.method]
+            t_gemm = (prof["rowpass"][0] + prof["colpass"][0]) / args.steps / 1e3
+            fl = mfma_flops(msgs, R, *terms)
+            roofline["mfma"] = {"issued_tflops": round(fl / t_gemm / 1e12, 2), "peak": BF16_PEAK_TFLOPS,
+                                "frac": round(fl / t_gemm / 1e12 / BF16_PEAK_TFLOPS, 4),
+                                "kernels": "rowpass+colpass"}
 
     cpu = None
     if not args.no_cpu:
@@ -186,6 +306,11 @@ def main():
                    imgs, n, busy)}
         print("cpu baseline: %d images in %.2f s on %d threads" % (imgs, busy, cores), file=sys.stderr, flush=True)
 
+    desc = {"bf16mix": "BF16 banded-Toeplitz GEMMs on MFMA (x pass bf16, y pass 3-term split), FP32 accumulate",
+            "bf16x3": "BF16 banded-Toeplitz GEMMs on MFMA, 3-term split, FP32 accumulate",
+            "bf16x2": "BF16 banded-Toeplitz GEMMs on MFMA, data split hi+lo, FP32 accumulate",
+            "bf16": "BF16 banded-Toeplitz GEMMs on MFMA, FP32 accumulate",
+            "fp32": "plain FP32"}[args.method]
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -197,13 +322,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16" if args.method != "fp32" else "f32",
         "data": "synthetic",
-        "config": {"workload": "plain FP32 sweep, 32 x (300x300 positions x 24 rotations x 1 scale), "
-                               "10-part tree, per-image spatial model",
-                   "instances": n, "height": H, "width": W, "rotations": ep.num_rotation_steps,
-                   "scales": ep.num_scale_steps, "parts": 10, "method": "plain-fp32",
-                   "execution": "hipgraph" if args.graph else "stream"},
+        "config": {"workload": "%s sweep, %d x (%dx%d positions x %d rotations x 1 scale), 10-part tree, "
+                               "per-image spatial model" % (desc, n, H, W, R),
+                   "instances": n, "height": H, "width": W, "rotations": R, "scales": ep.num_scale_steps,
+                   "parts": 10, "method": args.method, "execution": "hipgraph" if args.graph else "stream"},
+        "plain_fp32": fp32,
+        "vs_plain_fp32": round(value / fp32["value"], 3) if fp32 else None,
+        "error_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -128,16 +128,20 @@ int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, float *log_marg
  * FP32).  PS_METHOD_FP32 (default) is the drop-in: sequential FP32 dot
  * products in the reference's order.  The BF16 methods run both passes as
  * banded-Toeplitz GEMMs on the matrix cores with FP32 accumulation and a
- * power-of-two rescale per rotation slice: PS_METHOD_BF16 rounds data and
- * taps to bf16, PS_METHOD_BF16X3 splits both into bf16 hi + lo and adds three
- * products (about 16 significant bits).  Available for 24 rotations and half
- * widths up to 111; a level of messages outside that runs FP32
+ * power-of-two rescale per column of a rotation slice: PS_METHOD_BF16 rounds
+ * data and taps to bf16, PS_METHOD_BF16X2 splits the data into bf16 hi + lo
+ * (two products, taps in bf16), PS_METHOD_BF16X3 splits both and adds three
+ * products (about 16 significant bits), PS_METHOD_BF16_MIXED runs the x pass
+ * as PS_METHOD_BF16 and the y pass as PS_METHOD_BF16X3.  Available for 24 rotations and half
+ * widths up to 47 (sigma * scale up to 15.5 px); a level of messages outside that runs FP32
  * (ps_plan_method_levels counts the levels that use the method).  No
  * reference counterpart: a precision option of this library. */
 enum {
   PS_METHOD_FP32 = 0,
   PS_METHOD_BF16 = 1,
-  PS_METHOD_BF16X3 = 3
+  PS_METHOD_BF16X2 = 2,
+  PS_METHOD_BF16X3 = 3,
+  PS_METHOD_BF16_MIXED = 4
 };
 int ps_plan_set_method(ps_plan *plan, int method);
 int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);

@@ -212,11 +212,12 @@ class Plan:
 
     KERNEL_CLASSES = ("reset", "leaf_max", "rowpass", "colpass", "root", "map_decode")
 
-    METHODS = {"fp32": 0, "bf16": 1, "bf16x3": 3}
+    METHODS = {"fp32": 0, "bf16": 1, "bf16x2": 2, "bf16x3": 3, "bf16mix": 4}
 
     def set_method(self, method):
-        """Arithmetic of the spatial passes: 'fp32' (the drop-in), 'bf16' or
-        'bf16x3' (banded-Toeplitz GEMMs on the matrix cores).  Returns
+        """Arithmetic of the spatial passes: 'fp32' (the drop-in), 'bf16',
+        'bf16x2', 'bf16x3' or 'bf16mix' (banded-Toeplitz GEMMs on the matrix
+        cores; 'bf16mix' = x pass 'bf16', y pass 'bf16x3').  Returns
         (levels, levels that use the method)."""
         check(lib().ps_plan_set_method(self.handle, self.METHODS[method]), "ps_plan_set_method")
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -32,7 +32,7 @@ hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
-size_t rowpass_mfma_lds_bytes(int R, int W, int KK, int terms);
+size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms);
 size_t colpass_mfma_lds_bytes(int KK, int terms);
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes);
 hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
@@ -60,8 +60,8 @@ struct Level {
   size_t col_lds = 0;  // column pass
   long long od_cells = 0, od_hw = 0;  // max R*th*tw and H*W of the off-diagonal jobs
   bool mfma_ok = false;                // every diagonal job fits the BF16 Toeplitz passes
-  size_t mfma_row_lds[2] = {0, 0};     // 1 and 3 terms
-  size_t mfma_col_lds[2] = {0, 0};
+  size_t mfma_row_lds[3] = {0, 0, 0};  // 1, 2 and 3 terms
+  size_t mfma_col_lds[3] = {0, 0, 0};
 };
 
 // Host bf16 of a float, round to nearest even (finite inputs).
@@ -122,7 +122,8 @@ struct ps_plan {
   float *taps = nullptr;
   int *shifts = nullptr;
   uint4 *frags = nullptr;
-  int terms = 0;  // spatial filter: 0 FP32, 1 or 3 BF16 Toeplitz terms
+  int terms = 0;      // x pass: 0 FP32, else the BF16 Toeplitz terms (1, 2, 3)
+  int col_terms = 0;  // y pass
   float *scratch = nullptr;
   size_t scratch_elems = 0;
   unsigned *keys = nullptr;
@@ -539,14 +540,14 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         } else {
           ++L.n_diag;
           L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
-          if (j.kk_x < 1 || j.kk_y < 1) L.mfma_ok = false;
-          for (int t = 0; t < 2 && L.mfma_ok; ++t) {
-            L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kk_x, t ? 3 : 1));
-            L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t ? 3 : 1));
+          if (j.kk_x < 1 || j.kk_y < 1 || j.W > 512) L.mfma_ok = false;
+          for (int t = 0; t < 3 && L.mfma_ok; ++t) {
+            L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kx, j.kk_x, t + 1));
+            L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t + 1));
           }
         }
       }
-      if (L.mfma_row_lds[1] > kLdsLimit || L.mfma_col_lds[1] > kLdsLimit) L.mfma_ok = false;
+      if (L.mfma_row_lds[2] > kLdsLimit || L.mfma_col_lds[2] > kLdsLimit) L.mfma_ok = false;
       if (L.lds > kLdsLimit || L.col_lds > kLdsLimit)
         return set_error(PS_ERR_UNSUPPORTED, "row pass needs " + std::to_string(L.lds) +
                                                  " bytes of LDS (R * width too large for one workgroup)");
@@ -624,8 +625,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   for (auto *lv : {&pl->up, &pl->down})
     for (auto &L : *lv)
       if (L.mfma_ok) {
-        mrow = std::max(mrow, L.mfma_row_lds[1]);
-        mcol = std::max(mcol, L.mfma_col_lds[1]);
+        mrow = std::max(mrow, L.mfma_row_lds[2]);
+        mcol = std::max(mcol, L.mfma_col_lds[2]);
       }
   if (e == hipSuccess) e = mfma_set_lds_limit(mrow, mcol);
   if (e != hipSuccess) {
@@ -638,18 +639,21 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
 
 extern "C" int ps_plan_set_method(ps_plan *plan, int method) {
   if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
-  int terms;
+  int terms, col_terms;
   switch (method) {
-    case PS_METHOD_FP32: terms = 0; break;
-    case PS_METHOD_BF16: terms = 1; break;
-    case PS_METHOD_BF16X3: terms = 3; break;
+    case PS_METHOD_FP32: terms = col_terms = 0; break;
+    case PS_METHOD_BF16: terms = col_terms = 1; break;
+    case PS_METHOD_BF16X2: terms = col_terms = 2; break;
+    case PS_METHOD_BF16X3: terms = col_terms = 3; break;
+    case PS_METHOD_BF16_MIXED: terms = 1; col_terms = 3; break;
     default: return set_error(PS_ERR_INVALID_ARG, "unknown method " + std::to_string(method));
   }
-  if (terms != plan->terms && plan->gexec) {
+  if ((terms != plan->terms || col_terms != plan->col_terms) && plan->gexec) {
     (void)hipGraphExecDestroy(plan->gexec);
     plan->gexec = nullptr;
   }
   plan->terms = terms;
+  plan->col_terms = col_terms;
   return PS_OK;
 }
 
@@ -721,7 +725,8 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
   }
   auto level = [&](const Level &L) {
     const bool mf = pl->terms != 0 && L.mfma_ok;
-    const int ti = pl->terms == 3 ? 1 : 0;
+    const int ti = pl->terms > 0 ? pl->terms - 1 : 0;
+    const int tc = pl->col_terms > 0 ? pl->col_terms - 1 : 0;
     const MsgJob *jobs = pl->jobs + L.off;
     long ev2 = profile_begin(pp, 2, s);
     if (mf) {
@@ -738,8 +743,8 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     // the column pass class also times the eigenbasis stages
     ev2 = profile_begin(pp, 3, s);
     if (mf)
-      e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->terms, L.max_w,
-                              L.mfma_col_lds[ti], s);
+      e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->col_terms, L.max_w,
+                              L.mfma_col_lds[tc], s);
     else
       e = launch_colpass(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
     if (e == hipSuccess)

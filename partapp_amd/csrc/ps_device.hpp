@@ -65,15 +65,16 @@ struct DstAcc {
 };
 
 // The caller's accumulation of one message value msg at element
-// e = r*H*W + y*W + x of slice r.  pu is the unary of the part written (read
-// when need_u), pb the stored upward belief (read when need_b), xmask the
-// part's mask at (r, x).
+// flat = r*H*W + y*W + x of slice r, stored at dst[e] / dst2[e] (e = flat, or
+// y*W + x when the pointers already point at slice r).  pu is the unary of
+// the part written (read when need_u), pb the stored upward belief (read when
+// need_b), xmask the part's mask at (r, x).
 //   up, parent = root:  the message slot (objectdetect_findrot.cpp:637-649)
 //   up, chain node:     belief = 0 + message (+ unary) (:637, :652-654)
 //   down:               marginal = belief + message (:201), next downward
 //                       source = unary + message (:220-222)
-__device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *dst2, long long e, bool xmask,
-                                          float msg, float pu, float pb, DstAcc &a) {
+__device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *dst2, long long e, unsigned flat,
+                                          bool xmask, float msg, float pu, float pb, DstAcc &a) {
   if (J.dst_kind == kDstUpRoot) {
     dst[e] = msg;
   } else if (J.dst_kind == kDstUpNode) {
@@ -86,7 +87,7 @@ __device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *ds
     const float base = J.dst_base == 0 ? pb : (J.dst_base == 1 ? u : 0.0f);
     const float mg = base + msg;
     dst[e] = mg;
-    const unsigned long long k = amax_key(mg, (unsigned)e);
+    const unsigned long long k = amax_key(mg, flat);
     a.amax = k > a.amax ? k : a.amax;
     if (dst2) {
       const float v2 = u + msg;

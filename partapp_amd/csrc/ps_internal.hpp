@@ -12,7 +12,7 @@ constexpr int kMaxRot = 64;        // rotation masks are 64-bit words
 constexpr int kMaxRootChildren = 16;
 constexpr int kMaxTaps = 1000;     // multi_array_filter.hpp:243-244
 constexpr int kNoSourceRow = -(1 << 30);  // row offset of an output slice with no source slice
-constexpr int kMfmaMaxKK = 16;     // BF16 Toeplitz passes: half widths up to 111 (2k + 32 <= 16 * 16)
+constexpr int kMfmaMaxKK = 8;      // BF16 Toeplitz passes: half widths up to 47 (2k + 32 <= 8 * 16)
 
 // Which base pointer an offset is relative to.
 enum BufId : int { kBufUnary = 0, kBufMarginal = 1, kBufScratch = 2, kBufRoot = 3 };

@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
       // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
       const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(acc[i]) + M : kLogZero;
       const long long e = r * HW + (long long)y * W + x;
-      dst_store(J, dst, dst2, e, xmask, msg, need_u ? pu[i] : 0.0f, need_b ? pb[i] : 0.0f, da);
+      dst_store(J, dst, dst2, e, (unsigned)e, xmask, msg, need_u ? pu[i] : 0.0f, need_b ? pb[i] : 0.0f, da);
     }
     if (more) {
       // move rows [64, nrows) to [0, Lp), then store the prefetched rows

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_back(Bases b, const MsgJob *_
     const float pu = need_u ? dun[i] : 0.0f;
     const float pb = need_b ? dst[i] : 0.0f;
     const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
-    dst_store(J, dst, dst2, i, xmask, msg, pu, pb, da);
+    dst_store(J, dst, dst2, i, (unsigned)i, xmask, msg, pu, pb, da);
   }
   dst_flush(b, J, dst2 != nullptr, da, threadIdx.x & 63);
 }

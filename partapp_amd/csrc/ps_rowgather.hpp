@@ -89,4 +89,50 @@ __device__ __forceinline__ void row_column(const float *src, const int *oxo, con
   }
 }
 
+// The same in two halves, so a block can issue the loads of its next row
+// before it processes the current one.  row_loads fills v with the raw
+// source values of column x of row y and returns bit ro set when output slice
+// ro has a source pixel; row_finish applies the masks, exp(b - M) and the
+// rotation filter.
+template <int R>
+__device__ __forceinline__ unsigned long long row_loads(const float *src, const int *oxo, const int *oyo, int H,
+                                                        int W, int HW32, int delta, bool zero_src, int y, int x,
+                                                        float (&v)[R]) {
+  static_assert(R <= 64, "one bit per slice");
+  unsigned long long ok_bits = 0ull;
+  int off[R];
+#pragma unroll
+  for (int ro = 0; ro < R; ++ro) {
+    const int yy = y + oyo[ro];
+    const int xx = x + oxo[ro];
+    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    off[ro] = ok ? (ro - delta) * HW32 + yy * W + xx : 0;
+    ok_bits |= ok ? (1ull << ro) : 0ull;
+  }
+#pragma unroll
+  for (int ro = 0; ro < R; ++ro) v[ro] = zero_src ? 0.0f : src[off[ro]];
+  return ok_bits;
+}
+template <int R>
+__device__ __forceinline__ void row_finish(const float (&v)[R], unsigned long long ok_bits, const int *oxo,
+                                           const float *fr, float negM, float e_fill, int W, int delta, int rot_mode,
+                                           int rot_len, bool use_mask, const UnaryMask &mk, int x, float (&o)[R]) {
+  float e[R];
+#pragma unroll
+  for (int ro = 0; ro < R; ++ro) {
+    float vv = v[ro];
+    if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
+    e[ro] = ((ok_bits >> ro) & 1ull) ? expf(vv + negM) : e_fill;
+  }
+  if (rot_mode == kRotFilter) {
+    RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1>::run(rot_len, e, o, fr);
+  } else if (rot_mode == kRotCopy) {
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) o[ro] = e[ro];
+  } else {
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) o[ro] = 0.0f;
+  }
+}
+
 }  // namespace ps

@@ -62,7 +62,7 @@ def errors_vs_f64(model, u, joints, out):
     return worst, map_bad
 
 
-@pytest.mark.parametrize("method", ["bf16x3", "bf16"])
+@pytest.mark.parametrize("method", ["bf16x3", "bf16x2", "bf16mix", "bf16"])
 @pytest.mark.parametrize("seed", [0, 1])
 def test_bf16_sweep_error_vs_f64(gpu, method, seed):
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)

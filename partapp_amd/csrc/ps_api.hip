@@ -520,8 +520,6 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->items = items;
   pl->max_rows = max_rows;
   pl->max_hw = max_hw;
-  // the slice max keys of the BF16 passes follow the per-item keys
-  const int slice_key_base = items * 2 * P;
   auto add_levels = [&](std::vector<std::vector<MsgJob>> &lv, std::vector<Level> &dst) -> int {
     for (auto &v : lv) {
       std::stable_partition(v.begin(), v.end(), [](const MsgJob &j) { return j.od == 0; });
@@ -530,7 +528,6 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       L.count = (int)v.size();
       L.mfma_ok = R == 24;
       for (MsgJob &j : v) {
-        j.slice_key = slice_key_base + (int)(B.jobs.size() + (&j - v.data())) * R;
         L.max_h = std::max(L.max_h, j.H);
         L.max_w = std::max(L.max_w, j.W);
         L.lds = std::max(L.lds, rowpass_lds_bytes(R, j.W, j.kx));
@@ -564,7 +561,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   }
   pl->n_roots = (int)B.roots.size();
   pl->n_maxes = (int)B.maxes.size();
-  pl->n_keys = slice_key_base + (int)B.jobs.size() * R;
+  pl->n_keys = items * 2 * P;
   pl->n_map = items * P;
   pl->scratch_elems = scratch_off;
 

@@ -76,12 +76,9 @@ struct MsgJob {
   int dst2_key;
   int map_slot;       // Down: argmax key of the child marginal
   // BF16 Toeplitz variant of the x / y passes: k-steps of 16 and offsets (in
-  // 16-byte units) of the tap fragment tables (hi block, then lo block), and
-  // the first of R max keys of the row-filtered slices
+  // 16-byte units) of the tap fragment tables (hi block, then lo block)
   int kk_x, kk_y;
   int frag_x, frag_y;
-  int slice_key;
-  int pad2_;
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the

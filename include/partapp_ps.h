@@ -35,7 +35,9 @@ typedef enum ps_status {
   PS_ERR_WARP = -6,          /* a nearest-neighbour warp is not a uniform shift over the grid */
   PS_ERR_HIP = -7,           /* HIP runtime failure                          */
   PS_ERR_NO_DEVICE = -8,     /* no usable gfx950 device                      */
-  PS_ERR_OUT_OF_MEMORY = -9
+  PS_ERR_OUT_OF_MEMORY = -9,
+  PS_ERR_IO = -10,           /* file cannot be read / written, or is not a level-5 MAT file */
+  PS_ERR_NOT_FOUND = -11     /* no such MAT-file variable                    */
 } ps_status;
 
 /* ExpParam fields used on the path (libPartApp/ExpParam.proto:189-195,229);
@@ -202,6 +204,30 @@ int ps_find_local_max(const float *grids_dev, const long long *offsets, const in
  * per output cell for the scatter); synchronous. */
 int ps_ingest_scores(const float *score_grid_dev, int n, int grid_height, int grid_width, const double *Tig,
                      int interpolate, int height, int width, float *log_unary_dev, void *stream);
+
+/* --------------------------------------------------------- MAT-file (level 5)
+ * The on-disk formats around the path: joint_<child>_<parent>.mat
+ * (objectdetect_learnparam.cpp:61-179), the cell_scoregrid / transform_Ti2 /
+ * transform_T2g score-grid files (partapp.cpp:792-857) and the log_prob_grid /
+ * best_conf dumps (objectdetect_findrot.cpp:239-253, 1005-1011), read and written
+ * like matlab_io (matlab_io.hpp:64-230, matlab_cell_io.hpp:27-130): numeric
+ * arrays of any storage type are returned as double in C order; cell arrays
+ * are addressed by their column-major cell index.  Host code, no device. */
+typedef struct ps_mat ps_mat;
+int ps_mat_open(const char *path, ps_mat **out);
+void ps_mat_close(ps_mat *mat);
+/* Variable `name` (or element cell_index >= 0 of that cell array): dims in
+ * MATLAB order, and, when out is not NULL, numel values in C order. */
+int ps_mat_var(const ps_mat *mat, const char *name, long long cell_index, int *is_cell, int *ndim,
+               long long dims[8], double *out, long long capacity);
+/* Write (append != 0: add to an existing file) one real array given in C
+ * order as class double or single, optionally zlib-compressed (matOpen "wz"). */
+int ps_mat_write(const char *path, int append, const char *name, const double *data, int ndim,
+                 const long long *dims, int single, int compress);
+/* load_joint (objectdetect_learnparam.cpp:126-170) + the per-joint part of
+ * loadJoints (objectdetect_aux.cpp:96-136): flipped if asked, 0-based ids. */
+enum { PS_JOINT_POS_GAUSSIAN = 1, PS_JOINT_ROT_GAUSSIAN = 2 };
+int ps_load_joint_mat(const char *path, int flip, ps_joint *out, int *type);
 
 /* --------------------------------------------------------------- utilities */
 const char *ps_last_error_string(void);

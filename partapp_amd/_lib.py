@@ -119,6 +119,13 @@ SIGNATURES = {
                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_ingest_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ps_mat_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "ps_mat_close": (None, [ctypes.c_void_p]),
+    "ps_mat_var": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_longlong, c_int_p, c_int_p,
+                                  ctypes.POINTER(ctypes.c_longlong), c_double_p, ctypes.c_longlong]),
+    "ps_mat_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, c_double_p, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, ctypes.c_int]),
+    "ps_load_joint_mat": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ps_joint), c_int_p]),
     "ps_last_error_string": (ctypes.c_char_p, []),
     "ps_version": (ctypes.c_char_p, []),
     "ps_debug_message_plan": (ctypes.c_int, [ctypes.POINTER(ps_exp_param), ctypes.c_int, ctypes.c_int, c_double_p,

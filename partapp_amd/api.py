@@ -395,3 +395,128 @@ def config_exp_param(cfg: int) -> ExpParam:
     if cfg in (0, 1):
         return ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
     return ExpParam(-180.0, 180.0, 24, 0.7, 1.3, 3, 0.0)
+
+
+# ------------------------------------------------------------ MAT-file (level 5)
+
+class MatFile:
+    """A level-5 MAT file read into memory (ps_mat_open): variables by name,
+    cell-array elements by their column-major cell index, numeric data as
+    float64 numpy arrays in C order (matlab_io.hpp:154-230,
+    matlab_cell_io.hpp:27-130)."""
+
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        check(lib().ps_mat_open(str(path).encode(), ctypes.byref(h)), "ps_mat_open")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ps_mat_close(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _info(self, name, cell_index):
+        is_cell, nd = ctypes.c_int(), ctypes.c_int()
+        dims = (ctypes.c_longlong * 8)()
+        check(lib().ps_mat_var(self.handle, name.encode(), cell_index, ctypes.byref(is_cell), ctypes.byref(nd),
+                               dims, None, 0), "ps_mat_var")
+        return bool(is_cell.value), tuple(dims[i] for i in range(nd.value))
+
+    def shape(self, name, cell_index=-1):
+        return self._info(name, cell_index)[1]
+
+    def is_cell(self, name):
+        return self._info(name, -1)[0]
+
+    def get(self, name, cell_index=-1):
+        """float64 array in C order (a cell array needs a cell index)."""
+        _, shape = self._info(name, cell_index)
+        out = np.empty(shape, dtype=np.float64)
+        nd = ctypes.c_int()
+        dims = (ctypes.c_longlong * 8)()
+        check(lib().ps_mat_var(self.handle, name.encode(), cell_index, None, ctypes.byref(nd), dims,
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size), "ps_mat_var")
+        return out
+
+    def cells(self, name):
+        """A 2-D cell array as nested lists [i1][i2] (matlab_cell_io.hpp:54-64)."""
+        n1, n2 = self.shape(name)
+        return [[self.get(name, i1 + n1 * i2) for i2 in range(n2)] for i1 in range(n1)]
+
+
+def mat_write(path, name, array, append=False, single=None, compress=True):
+    """Write one real array (C order in memory; stored column-major) as class
+    double, or single for float32 data (matlab_io.hpp:64-129)."""
+    a = np.ascontiguousarray(np.asarray(array, dtype=np.float64))
+    if a.ndim == 0:
+        a = a.reshape(1, 1)
+    if single is None:
+        single = np.asarray(array).dtype == np.float32
+    dims = (ctypes.c_longlong * a.ndim)(*a.shape)
+    check(lib().ps_mat_write(str(path).encode(), int(bool(append)), name.encode(),
+                             a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.ndim, dims, int(bool(single)),
+                             int(bool(compress))), "ps_mat_write")
+
+
+JOINT_POS_GAUSSIAN, JOINT_ROT_GAUSSIAN = 1, 2
+
+
+def load_joint_mat(path, flip=False):
+    """load_joint + loadJoints' flip and 0-based ids for one joint_<c>_<p>.mat
+    (objectdetect_learnparam.cpp:126-170, objectdetect_aux.cpp:96-136).
+    Returns (Joint, type)."""
+    j = ps_joint()
+    t = ctypes.c_int()
+    check(lib().ps_load_joint_mat(str(path).encode(), int(bool(flip)), ctypes.byref(j), ctypes.byref(t)),
+          "ps_load_joint_mat")
+    return Joint.from_c(j), t.value
+
+
+def save_joint_mat(path, child_id, parent_id, offset_c, offset_p, C, rot_mean=0.0, rot_sigma=0.0,
+                   jtype=JOINT_ROT_GAUSSIAN):
+    """save_joint (objectdetect_learnparam.cpp:61-89): part ids (1-based),
+    scalars as 1x1, offsets as 2x1, C as 2x2, compressed."""
+    mat_write(path, "type", float(jtype))
+    mat_write(path, "child_idx", float(child_id), append=True)
+    mat_write(path, "parent_idx", float(parent_id), append=True)
+    mat_write(path, "offset_c", np.asarray(offset_c, np.float64).reshape(2, 1), append=True)
+    mat_write(path, "offset_p", np.asarray(offset_p, np.float64).reshape(2, 1), append=True)
+    mat_write(path, "C", np.asarray(C, np.float64).reshape(2, 2), append=True)
+    mat_write(path, "rot_mean", float(rot_mean), append=True)
+    mat_write(path, "rot_sigma", float(rot_sigma), append=True)
+
+
+def load_joints(spatial_dir, joint_ids, flip=False):
+    """loadJoints (objectdetect_aux.cpp:54-141) for the (child_id, parent_id)
+    part-id pairs of a PartConfig, from <spatial_dir>/joint_<c>_<p>.mat."""
+    import os
+    return [load_joint_mat(os.path.join(spatial_dir, "joint_%d_%d.mat" % (c, p)), flip)[0] for c, p in joint_ids]
+
+
+def save_part_marginal(path, log_prob_grid):
+    """The log_part_posterior_final_*.mat dump (objectdetect_findrot.cpp:239-253):
+    variable log_prob_grid, single [R][H][W]."""
+    mat_write(path, "log_prob_grid", np.asarray(log_prob_grid, np.float32), single=True)
+
+
+def load_part_marginal(path):
+    """PartApp::loadPartMarginal (partapp.cpp:807-827): float32 [R][H][W]."""
+    with MatFile(path) as m:
+        return m.get("log_prob_grid").astype(np.float32)
+
+
+def load_score_grid(path):
+    """The score-grid file of loadScoreGrid (partapp.cpp:829-857): the
+    cell_scoregrid cells [scale][rotation] as float32 grids and the
+    transform_Ti2 / transform_T2g matrices [scale][rotation][3][3]."""
+    with MatFile(path) as m:
+        grids = [[g.astype(np.float32) for g in row] for row in m.cells("cell_scoregrid")]
+        return grids, m.get("transform_Ti2"), m.get("transform_T2g")

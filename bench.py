@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--method", default="bf16mix", choices=["bf16mix", "bf16x3", "bf16x2", "bf16", "fp32"],
                     help="the chosen method (value); plain FP32 is timed beside it")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--check", type=int, default=2, help="instances checked against the FP64 oracle")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -252,6 +253,13 @@ def main():
         fp32 = {"value": round(n * args.steps / (ms32 / 1e3), 3), "ms_per_step": round(ms32 / args.steps, 4)}
         print("plain fp32: %.3f ms/step" % (ms32 / args.steps), file=sys.stderr, flush=True)
 
+    tuned = None
+    if args.method != "fp32" and not args.no_autotune:
+        plan.set_method(args.method)
+        times = plan.autotune(unary, marg, root, maps, stream=stream)
+        tuned = {"mfma_rows_per_block": min(times, key=times.get),
+                 "ms_per_sweep": {str(k): round(v, 3) for k, v in times.items()}}
+        print("autotune: %s" % tuned, file=sys.stderr, flush=True)
     ms, prof = time_method(plan, args.method, unary, marg, root, maps, stream, args)
     ms_per_step = ms / args.steps
     value = n * args.steps / (ms / 1e3)
@@ -328,7 +336,8 @@ def main():
         "config": {"workload": "%s sweep, %d x (%dx%d positions x %d rotations x 1 scale), 10-part tree, "
                                "per-image spatial model" % (desc, n, H, W, R),
                    "instances": n, "height": H, "width": W, "rotations": R, "scales": ep.num_scale_steps,
-                   "parts": 10, "method": args.method, "execution": "hipgraph" if args.graph else "stream"},
+                   "parts": 10, "method": args.method, "execution": "hipgraph" if args.graph else "stream",
+                   "autotune": tuned},
         "plain_fp32": fp32,
         "vs_plain_fp32": round(value / fp32["value"], 3) if fp32 else None,
         "error_vs_fp64": err,

@@ -146,6 +146,11 @@ enum {
   PS_METHOD_BF16_MIXED = 4
 };
 int ps_plan_set_method(ps_plan *plan, int method);
+/* Launch-shape knobs for an autotuner (results do not depend on them):
+ * PS_TUNE_MFMA_ROWS = output rows per block of the BF16 x pass (1..64,
+ * default 4).  No reference counterpart. */
+enum { PS_TUNE_MFMA_ROWS = 1 };
+int ps_plan_tune(ps_plan *plan, int knob, int value);
 int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
 
 /* Per-kernel timing of a plan's executions, for measurement: while enabled,

@@ -224,6 +224,33 @@ class Plan:
         check(lib().ps_plan_method_levels(self.handle, ctypes.byref(a), ctypes.byref(b)), "ps_plan_method_levels")
         return a.value, b.value
 
+    TUNE_MFMA_ROWS = 1
+
+    def tune(self, knob, value):
+        check(lib().ps_plan_tune(self.handle, knob, int(value)), "ps_plan_tune")
+
+    def autotune(self, unary, marg, root, maps, candidates=(2, 4, 8, 16), reps=2, stream=None):
+        """Pick the fastest rows-per-block of the BF16 x pass for this plan
+        and method by timing `reps` sweeps per candidate (HIP events on the
+        execution stream); sets it and returns {candidate: ms per sweep}."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        times = {}
+        for c in candidates:
+            self.tune(self.TUNE_MFMA_ROWS, c)
+            with torch.cuda.stream(s):
+                self.execute(unary, marg, root, maps, stream=s)  # first use of this shape
+                t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t0.record(s)
+                for _ in range(reps):
+                    self.execute(unary, marg, root, maps, stream=s)
+                t1.record(s)
+            t1.synchronize()
+            times[c] = t0.elapsed_time(t1) / reps
+        best = min(times, key=times.get)
+        self.tune(self.TUNE_MFMA_ROWS, best)
+        return times
+
     def set_profiling(self, enable=True):
         check(lib().ps_plan_set_profiling(self.handle, int(bool(enable))), "ps_plan_set_profiling")
 

@@ -36,8 +36,8 @@ size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms);
 size_t colpass_mfma_lds_bytes(int KK, int terms);
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes);
 hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                               const uint4 *frags, int R, int terms, int max_h, int max_w, size_t lds_bytes,
-                               hipStream_t s);
+                               const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
+                               size_t lds_bytes, hipStream_t s);
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
                                int R, int terms, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
@@ -124,6 +124,7 @@ struct ps_plan {
   uint4 *frags = nullptr;
   int terms = 0;      // x pass: 0 FP32, else the BF16 Toeplitz terms (1, 2, 3)
   int col_terms = 0;  // y pass
+  int mfma_rows = 4;  // output rows per block of the BF16 x pass (tunable)
   float *scratch = nullptr;
   size_t scratch_elems = 0;
   unsigned *keys = nullptr;
@@ -228,6 +229,7 @@ struct Builder {
     return off;
   }
   int add_message(const MessagePlan &mp, MsgJob &J) {
+    J.tpitch = t_pitch(J.W);
     J.delta = mp.delta;
     J.rot_mode = mp.rot_mode;
     J.rot_len = (int)mp.rot_taps.size();
@@ -307,8 +309,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   for (int i = 0; i < n_instances; ++i) {
     const ps_instance &in = instances[i];
     if (in.height < 1 || in.width < 1) return set_error(PS_ERR_INVALID_ARG, "instance sizes must be >= 1");
-    if ((long long)R * in.height * in.width >= (1ll << 31))
-      return set_error(PS_ERR_UNSUPPORTED, "R*H*W must be below 2^31");
+    if ((long long)R * in.height * t_pitch(in.width) >= (1ll << 31))
+      return set_error(PS_ERR_UNSUPPORTED, "R*H*W (rows padded to 32) must be below 2^31");
     if (in.unary_offset < 0 || in.marginal_offset < 0 || in.root_offset < 0)
       return set_error(PS_ERR_INVALID_ARG, "negative offset");
     if (in.joints) {
@@ -349,8 +351,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       auto unary = [&](int p) { return gref(kBufUnary, in.unary_offset + ((long long)p * S + s) * n); };
       auto marg = [&](int p) { return gref(kBufMarginal, in.marginal_offset + ((long long)p * S + s) * n); };
       const long long n64 = (n + 63) & ~63ll;  // grids start on 256-byte boundaries
-      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * n64); };
-      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)(K + k) * n64); };
+      const long long nt64 = ((long long)R * H * t_pitch(W) + 63) & ~63ll;  // row-pass outputs, padded rows
+      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * nt64); };
+      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + K * nt64 + (long long)k * n64); };
       auto up_key = [&](int p) { return it * 2 * P + p; };
       auto dn_key = [&](int p) { return it * 2 * P + P + p; };
       auto slot = [&](int p) { return it * P + p; };
@@ -394,7 +397,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       std::vector<long long> od_off(K, 0);
       long long od_total = 0;
       for (int k = 0; k < K; ++k) {
-        od_off[k] = (long long)scratch_off + 2ll * K * ((n + 63) & ~63ll) + od_total;
+        od_off[k] = (long long)scratch_off + K * nt64 + K * n64 + od_total;
         od_total += od_floats[k];
       }
       auto odbuf = [&](int k) { return gref(kBufScratch, od_off[k]); };
@@ -506,7 +509,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         mj.out = it * P + p;
         B.maps.push_back(mj);
       }
-      scratch_off += (size_t)(2 * K * n64 + od_total);
+      scratch_off += (size_t)(K * nt64 + K * n64 + od_total);
     }
   }
 
@@ -654,6 +657,22 @@ extern "C" int ps_plan_set_method(ps_plan *plan, int method) {
   return PS_OK;
 }
 
+extern "C" int ps_plan_tune(ps_plan *plan, int knob, int value) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  switch (knob) {
+    case PS_TUNE_MFMA_ROWS:
+      if (value < 1 || value > 64) return set_error(PS_ERR_INVALID_ARG, "rows per block must be in [1, 64]");
+      if (value != plan->mfma_rows && plan->gexec) {
+        (void)hipGraphExecDestroy(plan->gexec);
+        plan->gexec = nullptr;
+      }
+      plan->mfma_rows = value;
+      return PS_OK;
+    default:
+      return set_error(PS_ERR_INVALID_ARG, "unknown tuning knob " + std::to_string(knob));
+  }
+}
+
 extern "C" int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels) {
   if (!plan || !levels || !mfma_levels) return set_error(PS_ERR_INVALID_ARG, "null argument");
   *levels = (int)(plan->up.size() + plan->down.size());
@@ -727,8 +746,8 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     const MsgJob *jobs = pl->jobs + L.off;
     long ev2 = profile_begin(pp, 2, s);
     if (mf) {
-      e = launch_rowpass_mfma(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms, L.max_h, L.max_w,
-                              L.mfma_row_lds[ti], s);
+      e = launch_rowpass_mfma(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms, pl->mfma_rows,
+                              L.max_h, L.max_w, L.mfma_row_lds[ti], s);
       if (e == hipSuccess)
         e = launch_rowpass(b, jobs + L.n_diag, L.count - L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w,
                            L.lds, s);
@@ -888,7 +907,7 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   mj.dst = gref(kBufMarginal, 0);
   mj.dst_unary = kNone;
   mj.dst2 = kNone;
-  const long long n64 = ((long long)R * height * width + 63) & ~63ll;
+  const long long n64 = ((long long)R * height * t_pitch(width) + 63) & ~63ll;
   mj.od_buf = mp.od ? gref(kBufScratch, n64) : kNone;
   MaxJob xj;
   std::memset(&xj, 0, sizeof(xj));

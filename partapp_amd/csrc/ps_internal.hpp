@@ -63,7 +63,7 @@ struct MsgJob {
   int src_key;   // index of the source's max key
   GridRef src;   // [R][H][W]
   UnaryMask src_mask;
-  GridRef tmp;   // row-pass output [R][H][W]
+  GridRef tmp;   // row-pass output [R][H][tpitch]
   // destination
   int dst_kind;       // DstKind
   int dst_key;        // kDstUpNode: key of the parent belief (next source)
@@ -79,6 +79,8 @@ struct MsgJob {
   // 16-byte units) of the tap fragment tables (hi block, then lo block)
   int kk_x, kk_y;
   int frag_x, frag_y;
+  int tpitch;  // row pitch (floats) of the row-pass output grid, a multiple of 32
+  int pad2_;
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the
@@ -97,6 +99,8 @@ struct MsgJob {
 
 // floats of the eigenbasis scratch of one message
 inline long long od_scratch_floats(int R, int th, int tw) { return 3ll * R * th * tw; }
+// row pitch of the row-pass output: rows start on 128-byte boundaries
+inline int t_pitch(int W) { return (W + 31) & ~31; }
 
 // Root of one (instance, scale): sums the upward messages into the root
 // belief, forms the "all but one child" root messages

@@ -109,9 +109,10 @@ __device__ __forceinline__ void zero_halo(float *P, int R, int W, int kx, const 
 // truncated x filter (multi_array_filter.hpp:277-288) of every row of P:
 // eight outputs per lane, taps in chunks of eight (zero padded), a 16-float
 // window sliding along the row in two 16-byte LDS reads per chunk; writes
-// row y of T
-__device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x, float *T, int R, int W,
-                                             long long HW, int y, const RowGeom &g) {
+// row y of T (row pitch tp, a multiple of 32: the outputs past W land in the
+// padding)
+__device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x, float *T, int R, int tp,
+                                             long long plane, int y, const RowGeom &g) {
   const int n_items = R * g.Wo;
   const float inv_wo = 1.0f / (float)g.Wo;
   for (int item = threadIdx.x; item < n_items; item += blockDim.x) {
@@ -138,16 +139,10 @@ __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x
       w0 = w2;
       w1 = w3;
     }
-    float *trow = T + ro * HW + (long long)y * W;
+    float *trow = T + ro * plane + (long long)y * tp;
     const int x0 = 8 * q;
-    if ((W & 7) == 0) {
-      *reinterpret_cast<float4 *>(trow + x0) = make_float4(a[0], a[1], a[2], a[3]);
-      *reinterpret_cast<float4 *>(trow + x0 + 4) = make_float4(a[4], a[5], a[6], a[7]);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (x0 + i < W) trow[x0 + i] = a[i];
-    }
+    *reinterpret_cast<float4 *>(trow + x0) = make_float4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<float4 *>(trow + x0 + 4) = make_float4(a[4], a[5], a[6], a[7]);
   }
 }
 
@@ -224,7 +219,7 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
     for (int ro = 0; ro < R; ++ro) P[ro * g.Pw + kx + x] = o[ro];
   }
   __syncthreads();
-  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, W, HW, y, g);
+  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, J.tpitch, (long long)H * J.tpitch, y, g);
 }
 
 // Row pass for any rotation count: the exponentiated column goes through an
@@ -289,7 +284,7 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
     }
   }
   __syncthreads();
-  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, W, HW, y, g);
+  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, J.tpitch, (long long)H * J.tpitch, y, g);
 }
 
 // -------------------------------------------------------------- k_colpass
@@ -346,7 +341,8 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   const int xs = x + ox;
   const bool col_ok = xin && xs >= 0 && xs < W;
   const long long HW = (long long)H * W;
-  const float *Tr = cptr(b, J.tmp) + r * HW;
+  const int tp = J.tpitch;
+  const float *Tr = cptr(b, J.tmp) + (long long)r * H * tp;
   const int ky = J.ky;
   const int Lp = col_padded_taps(ky);
   const int nrows = col_tile_rows(ky);
@@ -371,7 +367,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   // addresses + select: the loads are unconditional).
   auto src_row = [&](int row) -> float {
     const bool in = lcol_ok && row >= 0 && row < H;
-    const float v = Tc[(long long)(in ? row : 0) * W];
+    const float v = Tc[(long long)(in ? row : 0) * tp];
     return in ? v : 0.0f;
   };
   constexpr int kNew = kColTileY / kColWaves;  // new rows per thread

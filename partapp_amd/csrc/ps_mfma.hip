@@ -138,6 +138,8 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   const UnaryMask mk = J.src_mask;
   const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
   const int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
+  const int tp = J.tpitch;
+  const int tplane = H * tp;
   const int x = threadIdx.x;
   const bool has_x = x < W;
   const int xl = has_x ? x : 0;
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
     }
     if (y + 1 < y1) ok = row_loads<R>(src, oxo, oyo, H, W, HW, J.delta, zero_src, y + 1, xl, v);
     __syncthreads();
-    float *T = mptr(b, J.tmp) + y * W;
+    float *T = mptr(b, J.tmp) + y * tp;
     for (int t = wave; t < ntile; t += nw) {
       const unsigned short *ah = Ph + m * pitch + (t << 5) + kq;
       const unsigned short *al = Pl + m * pitch + (t << 5) + kq;
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
         const int r = 8 * (i >> 2) + ((lane >> 5) << 2) + (i & 3);
         const float v = TERMS >= 2 ? c0[i] + c1[i] : c0[i];
         const int xs = xo - oxs[r < R ? r : 0];
-        if (r < R && xo < W && xs >= 0 && xs < W) T[r * HW + xs] = v;
+        if (r < R && xo < W && xs >= 0 && xs < W) T[r * tplane + xs] = v;
       }
     }
     __syncthreads();  // P is rewritten by the next row
@@ -240,7 +242,8 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
   // staging: this thread loads column xt * 64 + lane of the shifted slice, 16
   // consecutive rows at a time (rows outside the grid are zero)
   const int lcol = xt * 64 + lane;
-  const float *Tc = cptr(b, J.tmp) + r * HW + (lcol < W ? lcol : 0);
+  const int tp = J.tpitch;
+  const float *Tc = cptr(b, J.tmp) + (long long)r * H * tp + (lcol < W ? lcol : 0);
   const bool lcol_ok = lcol < W;
   float scale = 1.0f;
   auto load16 = [&](int row0, float (&v)[16]) {
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
     for (int k = 0; k < 16; ++k) {
       const int row = row0 + k;
       const bool in = all_in || (lcol_ok && row >= 0 && row < H);
-      const float t = Tc[(in ? row : 0) * W];
+      const float t = Tc[(in ? row : 0) * tp];
       v[k] = in ? t : 0.0f;
     }
   };
@@ -386,16 +389,12 @@ hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   return e;
 }
 
-// rows of one row-pass block of the MFMA path
-constexpr int kMfmaRowsPerBlock = 4;
-
 hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                               const uint4 *frags, int R, int terms, int max_h, int max_w, size_t lds_bytes,
-                               hipStream_t s) {
+                               const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
+                               size_t lds_bytes, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
   // one lane per column: the plan only routes grids of width <= 512 here
-  if (R != 24 || terms < 1 || terms > 3 || max_w > 512) return hipErrorInvalidValue;
-  const int rows = kMfmaRowsPerBlock;
+  if (R != 24 || terms < 1 || terms > 3 || max_w > 512 || rows < 1) return hipErrorInvalidValue;
   dim3 grid((max_h + rows - 1) / rows, n_jobs);
   const int nt = std::min(512, std::max(64, (max_w + 63) / 64 * 64));
   if (terms == 3)

@@ -36,10 +36,11 @@ __device__ __forceinline__ float *od_x(const Bases &b, const MsgJob &J, int R) {
   return mptr(b, J.od_buf) + 2ll * R * J.th * J.tw;
 }
 
-// TM_BILINEAR sample of a [h][w] grid at (x1, y1) (multi_array_transform.hpp:
-// 218-236): within 10 float epsilons of a grid point the point itself, the
-// last row and column (no right / lower neighbour) and outside: 0
-__device__ __forceinline__ float bilinear_at(const float *g, int h, int w, double x1, double y1) {
+// TM_BILINEAR sample of a [h][w] grid (row pitch `pitch`) at (x1, y1)
+// (multi_array_transform.hpp:218-236): within 10 float epsilons of a grid
+// point the point itself, the last row and column (no right / lower
+// neighbour) and outside: 0
+__device__ __forceinline__ float bilinear_at(const float *g, int h, int w, int pitch, double x1, double y1) {
   const int ix = (int)floor(x1);
   const int iy = (int)floor(y1);
   float r = 0.0f;
@@ -47,14 +48,14 @@ __device__ __forceinline__ float bilinear_at(const float *g, int h, int w, doubl
     const float eps10 = 10.0f * FLT_EPSILON;
     const float a = (float)(x1 - (double)ix);
     const float bb = (float)(y1 - (double)iy);
-    const float *p = g + (long long)iy * w + ix;
+    const float *p = g + (long long)iy * pitch + ix;
     if (a < eps10 && bb < eps10) {
       r = p[0];
     } else if (ix < w - 1 && iy < h - 1) {
       const float t0 = (1.0f - bb) * (1.0f - a) * p[0];
       const float t1 = (1.0f - bb) * a * p[1];
-      const float t2 = bb * (1.0f - a) * p[w];
-      const float t3 = bb * a * p[w + 1];
+      const float t2 = bb * (1.0f - a) * p[pitch];
+      const float t3 = bb * a * p[pitch + 1];
       r = t0 + t1 + t2 + t3;
     }
   }
@@ -86,13 +87,14 @@ __global__ __launch_bounds__(kOdThreads) void k_od_scatter(Bases b, const MsgJob
   unsigned long long *K = od_keys(b, J);
   const double m00 = J.t_fwd[0], m01 = J.t_fwd[1], m02 = J.t_fwd[2];
   const double m10 = J.t_fwd[3], m11 = J.t_fwd[4], m12 = J.t_fwd[5];
+  const int tp = J.tpitch;
   for (long long i = (long long)blockIdx.x * kOdThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kOdThreads) {
-    const float v = T[i];
-    if (v == 0.0f) continue;  // the default value is not written
     const int r = (int)(i / HW);
     const int rem = (int)(i - r * HW);
     const int y1 = rem / W;
     const int x1 = rem - y1 * W;
+    const float v = T[((long long)r * H + y1) * tp + x1];
+    if (v == 0.0f) continue;  // the default value is not written
     const double x3 = m00 * (double)x1 + m01 * (double)y1 + m02;
     const double y3 = m10 * (double)x1 + m11 * (double)y1 + m12;
     const int ix = (int)floor(x3 + 0.5);
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_gather(Bases b, const MsgJob 
     const int x3 = rem - y3 * tw;
     const double x1 = m00 * (double)x3 + m01 * (double)y3 + m02;
     const double y1 = m10 * (double)x3 + m11 * (double)y3 + m12;
-    A[i] = bilinear_at(T + (long long)r * H * W, H, W, x1, y1);
+    A[i] = bilinear_at(T + (long long)r * H * J.tpitch, H, W, J.tpitch, x1, y1);
   }
 }
 
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_back(Bases b, const MsgJob *_
     if (xs >= 0 && xs < W && ys >= 0 && ys < H) {
       const double xe = J.t_back[0] * (double)xs + J.t_back[1] * (double)ys + J.t_back[2];
       const double ye = J.t_back[3] * (double)xs + J.t_back[4] * (double)ys + J.t_back[5];
-      const float v = bilinear_at(od_a(b, J) + (long long)r * th * tw, th, tw, xe, ye);
+      const float v = bilinear_at(od_a(b, J) + (long long)r * th * tw, th, tw, tw, xe, ye);
       msg = log0(v) + M;
     }
     const bool need_u = (J.dst_kind == kDstUpNode && J.dst_add_unary) || J.dst_kind == kDstDown;

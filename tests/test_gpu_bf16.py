@@ -133,3 +133,30 @@ def test_method_fallback_for_wide_filters(gpu):
     assert used < levels
     worst, map_bad = errors_vs_f64(model, u, joints, outs[0])
     assert worst <= BUDGET_L1 and not map_bad
+
+
+def test_rows_per_block_knob_does_not_change_results(gpu):
+    import torch
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(8)
+    model = person_model(ep, joints)
+    u = synth_unary(8, 10, 1, 24, 100, 116)  # H = 100: the last block of 3 / 16 rows is partial
+    plan = Plan(model, [(100, 116)], [joints])
+    plan.set_method("bf16mix")
+    un = torch.from_numpy(u.ravel().copy()).to(gpu)
+    outs = []
+    for rows in (1, 3, 4, 16):
+        plan.tune(plan.TUNE_MFMA_ROWS, rows)
+        m = torch.full_like(un, float("nan"))
+        r = torch.full((plan.root_numel,), float("nan"), device=gpu)
+        maps = torch.zeros((plan.map_numel * 4,), dtype=torch.int32, device=gpu)
+        plan.execute(un, m, r, maps)
+        torch.cuda.synchronize()
+        outs.append((m.cpu(), r.cpu(), maps.cpu()))
+    from partapp_amd._lib import PsError
+    with pytest.raises(PsError):
+        plan.tune(plan.TUNE_MFMA_ROWS, 0)
+    plan.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)

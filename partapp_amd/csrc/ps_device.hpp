@@ -56,6 +56,9 @@ __device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) 
 
 // log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166)
 __device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
+// the same with the hardware logarithm (v_log_f32 times ln 2), for the BF16
+// passes whose inputs carry 8-bit rounding already
+__device__ __forceinline__ float log0_fast(float v) { return v == 0.0f ? kLogZero : __logf(v); }
 
 // Reductions a message's epilogue feeds: the max of the next upward source,
 // the argmax of a marginal and the max of the next downward source.

@@ -15,7 +15,8 @@
 // splits both and adds hi*hi + lo*hi + hi*lo.  The column pass rescales every
 // column of a slice by a power of two taken from its first rows (the y filter
 // never mixes columns), so the bf16 operands keep their range; the scale is
-// undone exactly after the GEMM.
+// undone exactly after the GEMM.  The exponential of the row pass and the
+// logarithm of the column pass use the hardware instructions (a few ulp).
 //
 // Operand layouts of the 32x32x16 bf16 MFMA (lane l, t = 0..7):
 //   A: A[l % 32][(l / 32) * 8 + t]   B: B[(l / 32) * 8 + t][l % 32]
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   for (int y = y0; y < y1; ++y) {
     if (has_x) {
       float o[R];
-      row_finish<R>(v, ok, oxo, fr_rot, negM, e_fill, W, J.delta, J.rot_mode, J.rot_len, use_mask, mk, x, o);
+      row_finish<R, true>(v, ok, oxo, fr_rot, negM, e_fill, W, J.delta, J.rot_mode, J.rot_len, use_mask, mk, x, o);
 #pragma unroll
       for (int ro = 0; ro < R; ++ro) Ph[ro * pitch + kx + x] = bf16_bits(o[ro]);
       if (TERMS >= 2) {
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
       if (!xin || y >= H) continue;
       const int ysrc = y + oy;
       const float v = TERMS >= 2 ? c0[i] + c1[i] : c0[i];
-      const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(v * unscale) + M : kLogZero;
+      const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0_fast(v * unscale) + M : kLogZero;
       dst_store(J, dst, dst2 ? dst2 + r * HW : nullptr, y * W + x, r * HW + y * W + x, xmask, msg, pu[i], pb[i],
                 da);
     }

@@ -93,7 +93,9 @@ __device__ __forceinline__ void row_column(const float *src, const int *oxo, con
 // before it processes the current one.  row_loads fills v with the raw
 // source values of column x of row y and returns bit ro set when output slice
 // ro has a source pixel; row_finish applies the masks, exp(b - M) and the
-// rotation filter.
+// rotation filter.  FAST selects the hardware exponential (v_exp_f32 of
+// x * log2(e), a few ulp) instead of libm's expf: for the BF16 passes, whose
+// operands are rounded to 8 bits right after.
 template <int R>
 __device__ __forceinline__ unsigned long long row_loads(const float *src, const int *oxo, const int *oyo, int H,
                                                         int W, int HW32, int delta, bool zero_src, int y, int x,
@@ -113,7 +115,7 @@ __device__ __forceinline__ unsigned long long row_loads(const float *src, const 
   for (int ro = 0; ro < R; ++ro) v[ro] = zero_src ? 0.0f : src[off[ro]];
   return ok_bits;
 }
-template <int R>
+template <int R, bool FAST = false>
 __device__ __forceinline__ void row_finish(const float (&v)[R], unsigned long long ok_bits, const int *oxo,
                                            const float *fr, float negM, float e_fill, int W, int delta, int rot_mode,
                                            int rot_len, bool use_mask, const UnaryMask &mk, int x, float (&o)[R]) {
@@ -122,7 +124,7 @@ __device__ __forceinline__ void row_finish(const float (&v)[R], unsigned long lo
   for (int ro = 0; ro < R; ++ro) {
     float vv = v[ro];
     if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
-    e[ro] = ((ok_bits >> ro) & 1ull) ? expf(vv + negM) : e_fill;
+    e[ro] = ((ok_bits >> ro) & 1ull) ? (FAST ? __expf(vv + negM) : expf(vv + negM)) : e_fill;
   }
   if (rot_mode == kRotFilter) {
     RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1>::run(rot_len, e, o, fr);

@@ -46,8 +46,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--instances", type=int, default=32)
-    ap.add_argument("--size", type=int, default=300)
+    ap.add_argument("--config", type=int, default=1, choices=[1, 2],
+                    help="1: 32 x 300x300x24, 1 scale (the metric's configuration); 2: 256 instances of uneven "
+                         "size (H, W in 160..480, multiples of 8) x 24 rotations x 3 scales")
+    ap.add_argument("--instances", type=int, default=None, help="default 32 (config 1) / 256 (config 2)")
+    ap.add_argument("--size", type=int, default=300, help="H = W of config 1")
     ap.add_argument("--method", default="bf16mix", choices=["bf16mix", "bf16x3", "bf16x2", "bf16", "fp32"],
                     help="the chosen method (value); plain FP32 is timed beside it")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison")
@@ -181,33 +184,33 @@ def time_method(plan, method, unary, marg, root, maps, stream, args):
     return t0.elapsed_time(t1), prof
 
 
-def check_vs_f64(plan, model, marg, maps, ep, joints, seeds, sizes, k):
-    """Relative L1 (normalised marginals, per part) and MAP agreement of the
-    last executed sweep against the FP64 oracle on the first k instances."""
+def check_vs_f64(plan, model, unary, marg, maps, ep, joints, k):
+    """Relative L1 (normalised marginals, per part and scale) and MAP
+    agreement of the last executed sweep against the FP64 oracle on the first
+    k instances, with the very unaries the GPU read."""
     import numpy as np
 
     import oracle
-    from partapp_amd.api import decode_maps, synth_unary
+    from partapp_amd.api import decode_maps
     oep = oracle.exp_param_from(ep)
     m = decode_maps(maps, plan.n_instances, plan.S, plan.P)
     cores = min(os.cpu_count() or 1, 16)
+    us = [plan.instance_view(unary, i).cpu().numpy().astype(np.float64) for i in range(k)]
 
     def one(i):
-        H, W = sizes[i]
-        u = synth_unary(seeds[i], 10, 1, ep.num_rotation_steps, H, W, threads=1)
-        return oracle.root_posterior_rot(oep, model.is_detect, model.is_upright, model.root, joints[i],
-                                         u.astype(np.float64))
+        return oracle.root_posterior_rot(oep, model.is_detect, model.is_upright, model.root, joints[i], us[i])
 
     worst, agree, total = 0.0, 0, 0
     with cf.ThreadPoolExecutor(max_workers=min(k, cores)) as ex:
         refs = list(ex.map(one, range(k)))
     for i, (r64, p64, b64, s64) in enumerate(refs):
         mg = plan.instance_view(marg, i).cpu().numpy()
-        for p in range(plan.P):
-            worst = max(worst, oracle.rel_l1(oracle.normalized(mg[p, 0]), oracle.normalized(p64[p, 0])))
-            got = (int(m[i, 0, p]["rotidx"]), int(m[i, 0, p]["y"]), int(m[i, 0, p]["x"]))
-            agree += int(got == tuple(b64[0, p]))
-            total += 1
+        for s in range(plan.S):
+            for p in range(plan.P):
+                worst = max(worst, oracle.rel_l1(oracle.normalized(mg[p, s]), oracle.normalized(p64[p, s])))
+                got = (int(m[i, s, p]["rotidx"]), int(m[i, s, p]["y"]), int(m[i, s, p]["x"]))
+                agree += int(got == tuple(b64[s, p]))
+                total += 1
     return worst, agree, total
 
 
@@ -222,24 +225,37 @@ def main():
 
     from partapp_amd.api import Plan, config_exp_param, person_model, synth_joints, synth_unary
 
+    from partapp_amd.api import synth_unary_dev
+
     dev = torch.device("cuda:0")
-    n, H = args.instances, args.size
-    W = H
-    ep = config_exp_param(1)
-    R = ep.num_rotation_steps
+    ep = config_exp_param(args.config)
+    R, S = ep.num_rotation_steps, ep.num_scale_steps
+    if args.config == 1:
+        n = args.instances or 32
+        sizes = [(args.size, args.size)] * n
+    else:
+        n = args.instances or 256
+        rng = np.random.default_rng(20131201)
+        sizes = [(8 * int(rng.integers(20, 61)), 8 * int(rng.integers(20, 61))) for _ in range(n)]
+    H, W = sizes[0]
     seeds = list(range(n))
     joints = [synth_joints(s) for s in seeds]
-    sizes = [(H, W)] * n
     model = person_model(ep, joints[0])
     plan = Plan(model, sizes, joints)
-    print("plan: %d instances %dx%dx%d, scratch %.2f GB" % (n, H, W, R, plan.scratch_bytes() / 1e9),
-          file=sys.stderr, flush=True)
+    states = sum(S * R * h * w for h, w in sizes)
+    print("plan: %d instances, %d scales, %.1f M states per part, scratch %.2f GB" % (
+        n, S, states / 1e6, plan.scratch_bytes() / 1e9), file=sys.stderr, flush=True)
     unary = torch.empty((plan.unary_numel,), dtype=torch.float32, device=dev)
     threads = min(os.cpu_count() or 1, 16)
     for i, s in enumerate(seeds):
-        u = synth_unary(s, 10, 1, R, H, W, threads=threads)
+        h, w = sizes[i]
         o = plan.unary_offsets[i]
-        unary[o:o + u.size].copy_(torch.from_numpy(u.ravel()))
+        numel = 10 * S * R * h * w
+        if args.config == 1:
+            u = synth_unary(s, 10, S, R, h, w, threads=threads)
+            unary[o:o + numel].copy_(torch.from_numpy(u.ravel()))
+        else:
+            synth_unary_dev(s, 10, S, R, h, w, unary[o:o + numel])
     marg = torch.empty_like(unary)
     root = torch.empty((plan.root_numel,), dtype=torch.float32, device=dev)
     maps = torch.zeros((plan.map_numel * 4,), dtype=torch.int32, device=dev)
@@ -268,13 +284,12 @@ def main():
     err = None
     if args.check > 0:
         k = min(args.check, n)
-        worst, agree, total = check_vs_f64(plan, model, marg, maps, ep, joints, seeds, sizes, k)
+        worst, agree, total = check_vs_f64(plan, model, unary, marg, maps, ep, joints, k)
         err = {"max_rel_l1": float("%.3g" % worst), "map_agreement": "%d/%d" % (agree, total),
                "sample": "%d of %d instances vs the FP64 oracle" % (k, n)}
         print("error vs fp64: %s" % err, file=sys.stderr, flush=True)
 
     msgs = message_plans(sizes, joints, ep)
-    grid_bytes = 4.0 * R * H * W
     roofline = None
     if prof is not None:
         for k, (t, c) in prof.items():
@@ -282,7 +297,7 @@ def main():
         dom = max(("rowpass", "colpass", "root"), key=lambda k: prof[k][0])
         t_dom, n_dom = prof[dom]
         traffic_grids = grid_traffic(model)
-        bytes_per_step = traffic_grids[dom] * grid_bytes * n
+        bytes_per_step = traffic_grids[dom] * 4.0 * states
         launches_per_step = n_dom / args.steps
         avg_ms = t_dom / n_dom
         achieved = bytes_per_step / launches_per_step / (avg_ms / 1e3) / 1e9
@@ -291,7 +306,7 @@ def main():
                     "avg_launch_ms": round(avg_ms, 4),
                     "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()}}
         # the whole sweep against the algorithmic minimum B_alg = 4 B * N_s * (2P + 4E)
-        b_alg = 4.0 * R * H * W * (2 * 10 + 4 * 9) * n
+        b_alg = 4.0 * states * (2 * 10 + 4 * 9)
         roofline["sweep_gbs"] = round(b_alg / (ms_per_step / 1e3) / 1e9, 1)
         if args.method == "fp32":
             row_f, col_f = workload_flops(msgs, R)
@@ -333,9 +348,14 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16" if args.method != "fp32" else "f32",
         "data": "synthetic",
-        "config": {"workload": "%s sweep, %d x (%dx%d positions x %d rotations x 1 scale), 10-part tree, "
-                               "per-image spatial model" % (desc, n, H, W, R),
-                   "instances": n, "height": H, "width": W, "rotations": R, "scales": ep.num_scale_steps,
+        "config": {"workload": ("%s sweep, %d x (%dx%d positions x %d rotations x 1 scale), 10-part tree, "
+                                "per-image spatial model" % (desc, n, H, W, R)) if args.config == 1 else
+                               ("%s sweep, %d instances of uneven size (H, W in 160..480, multiples of 8) x %d "
+                                "rotations x %d scales (0.8/1.0/1.2), 10-part tree, per-image spatial model"
+                                % (desc, n, R, S)),
+                   "instances": n, "height": H if args.config == 1 else "160..480",
+                   "width": W if args.config == 1 else "160..480", "rotations": R, "scales": ep.num_scale_steps,
+                   "states_per_part": states,
                    "parts": 10, "method": args.method, "execution": "hipgraph" if args.graph else "stream",
                    "autotune": tuned},
         "plain_fp32": fp32,

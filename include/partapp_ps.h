@@ -257,6 +257,10 @@ int ps_debug_message_plan(const ps_exp_param *ep, int height, int width, const d
 int ps_synth_joints(unsigned long long seed, int tree, int zero_rot_mean, ps_joint *joints, int *num_joints);
 int ps_synth_unary(unsigned long long seed, int num_parts, int num_scales, int num_rot, int height, int width,
                    float *unary, int n_threads);
+/* The same generator on the device (tens of GB of batches): writes the
+ * [P][S][R][H][W] grid at unary_dev; synchronous on `stream`.  Values equal
+ * the host generator's up to the device's double-precision libm. */
+int ps_synth_unary_dev(unsigned long long seed, int P, int S, int R, int H, int W, float *unary_dev, void *stream);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,8 @@ SIGNATURES = {
     "ps_mat_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, c_double_p, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, ctypes.c_int]),
     "ps_load_joint_mat": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ps_joint), c_int_p]),
+    "ps_synth_unary_dev": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_last_error_string": (ctypes.c_char_p, []),
     "ps_version": (ctypes.c_char_p, []),
     "ps_debug_message_plan": (ctypes.c_int, [ctypes.POINTER(ps_exp_param), ctypes.c_int, ctypes.c_int, c_double_p,

@@ -405,6 +405,15 @@ def synth_unary(seed, P, S, R, H, W, threads=8):
     return out
 
 
+def synth_unary_dev(seed, P, S, R, H, W, out, stream=None):
+    """ps_synth_unary on the device into `out`, a torch CUDA float32 tensor (or
+    a view) of P*S*R*H*W contiguous elements."""
+    assert out.is_contiguous() and out.numel() == P * S * R * H * W
+    check(lib().ps_synth_unary_dev(int(seed), P, S, R, H, W, out.data_ptr(), _stream_handle(stream)),
+          "ps_synth_unary_dev")
+    return out
+
+
 def person_model(exp_param: ExpParam, joints: Optional[Sequence[Joint]] = None, is_detect=None, is_upright=None,
                  is_sparse=True, seed=0) -> PartModel:
     """The 10-part person tree rooted at the torso (part 4)."""

@@ -276,3 +276,15 @@ def test_plan_errors_on_gpu(gpu):
     assert e.value.status == -3
     with pytest.raises(PsError):
         Plan(model, [(0, 20)])
+
+
+def test_device_generator_matches_host(gpu):
+    import torch
+    from partapp_amd.api import synth_unary_dev
+    u = synth_unary(13, 3, 2, 24, 37, 53)
+    d = torch.empty(u.size, dtype=torch.float32, device=gpu)
+    synth_unary_dev(13, 3, 2, 24, 37, 53, d)
+    g = d.cpu().numpy().reshape(u.shape)
+    # the device's double-precision log / sin / cos may differ in the last bit
+    assert np.mean(g == u) > 0.999
+    np.testing.assert_allclose(g, u, rtol=0, atol=1e-6)

@@ -25,6 +25,21 @@ __device__ __forceinline__ unsigned long long amax_key(float v, unsigned idx) {
   return ((unsigned long long)f2key(v) << 32) | (unsigned long long)(0xffffffffu - idx);
 }
 
+// The block of a 2-D grid (x fastest) renumbered so that each of the 8 XCDs,
+// to which the dispatcher hands workgroups round-robin in launch order, gets
+// one contiguous range of logical blocks.  Neighbouring tiles (which share the
+// cache lines at their edges when grid rows are not 128-byte aligned) then run
+// at the same time behind the same L2.
+__device__ __forceinline__ void xcd_block(unsigned &bx, unsigned &by) {
+  const unsigned n = gridDim.x * gridDim.y;
+  const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned xcd = b & 7u, k = b >> 3;
+  const unsigned q = n >> 3, rem = n & 7u;
+  const unsigned l = xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+  by = l / gridDim.x;
+  bx = l - by * gridDim.x;
+}
+
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o, 64));

@@ -154,8 +154,10 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
                                                    const float *__restrict__ taps, const int *__restrict__ shifts) {
   extern __shared__ float4 lds4[];
   float *P = reinterpret_cast<float *>(lds4);
-  const MsgJob &J = jobs[blockIdx.y];
-  const int y = blockIdx.x;
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
+  const int y = (int)bx;
   const int H = J.H, W = J.W;
   if (y >= H) return;
   const int kx = J.kx;
@@ -228,8 +230,10 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
                                                      const float *__restrict__ taps, const int *__restrict__ shifts,
                                                      int R) {
   extern __shared__ float4 lds4[];
-  const MsgJob &J = jobs[blockIdx.y];
-  const int y = blockIdx.x;
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
+  const int y = (int)bx;
   const int H = J.H, W = J.W;
   if (y >= H) return;
   const int kx = J.kx;
@@ -325,10 +329,12 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
                                                  int R, int xtiles) {
   extern __shared__ float4 lds4[];
   float *tile = reinterpret_cast<float *>(lds4);
-  const MsgJob &J = jobs[blockIdx.y];
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
   const int H = J.H, W = J.W;
-  const int r = blockIdx.x / xtiles;
-  const int xt = blockIdx.x - r * xtiles;
+  const int r = (int)bx / xtiles;
+  const int xt = (int)bx - r * xtiles;
   if (xt * kColTileX >= W) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   // column loaded by this thread's lane slot, and whether it exists
   const int lcol = xt * kColTileX + lane + ox;
   const bool lcol_ok = x < W && lcol >= 0 && lcol < W;
-  const float *Tc = Tr + (lcol_ok ? lcol : 0);
+  const float *Tc = Tr + min(max(lcol, 0), W - 1);  // loads outside the grid go to its nearest column
 
   DstAcc da;
 
@@ -367,7 +373,7 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
   // addresses + select: the loads are unconditional).
   auto src_row = [&](int row) -> float {
     const bool in = lcol_ok && row >= 0 && row < H;
-    const float v = Tc[(long long)(in ? row : 0) * tp];
+    const float v = Tc[(long long)(in ? row : min(max(row, 0), H - 1)) * tp];
     return in ? v : 0.0f;
   };
   constexpr int kNew = kColTileY / kColWaves;  // new rows per thread

@@ -58,6 +58,14 @@ __device__ __forceinline__ void split8(const float (&v)[8], bf16x8_t &hi, bf16x8
   }
 }
 
+// element at a 32-bit byte offset from a (wave-uniform) base
+__device__ __forceinline__ float &at(float *p, unsigned byte_off) {
+  return *reinterpret_cast<float *>(reinterpret_cast<char *>(p) + byte_off);
+}
+__device__ __forceinline__ float at(const float *p, unsigned byte_off) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+
 // 2^-e and 2^e that bring the maximum mx into [0.5, 1), with e limited to
 // [-60, 60] (both 1 when mx is 0 or not finite)
 __device__ __forceinline__ void slice_scale(float mx, float &scale, float &unscale) {
@@ -90,20 +98,19 @@ __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ri
 // P[r][kx + x] in LDS, while the loads of the next row are already in
 // flight; then each wave takes 32-wide column tiles and runs the KK-step GEMM
 // P (R x K, rows clamped to R - 1 beyond R) times the Toeplitz fragments, one
-// accumulator for hi*hi and one for the two cross terms.  The output of slice
-// r is stored shifted by the inverse warp's column offset (column x at
-// x - inv_ox[r]), so the column pass reads 64-column strips aligned with its
-// outputs; columns that no output reads are left unwritten.  The row pass
-// works on exp(b - M) as the FP32 pass does (M the source grid's maximum).
+// accumulator for hi*hi and one for the two cross terms.  The row pass works
+// on exp(b - M) as the FP32 pass does (M the source grid's maximum).
 template <int R, int TERMS>
 __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
                                                       const float *__restrict__ taps, const int *__restrict__ shifts,
                                                       const uint4 *__restrict__ frags, int rows) {
   static_assert(R <= 32, "one 32-row MFMA tile holds every rotation slice");
   extern __shared__ uint4 lds[];
-  const MsgJob &J = jobs[blockIdx.y];
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
   const int H = J.H, W = J.W;
-  const int y0 = blockIdx.x * rows;
+  const int y0 = (int)bx * rows;
   if (y0 >= H) return;
   const int y1 = min(H, y0 + rows);
   const int kx = J.kx, KK = J.kk_x;
@@ -113,11 +120,9 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   uint4 *fr = lds;  // [nf][KK][64] Toeplitz fragments
   unsigned short *Ph = reinterpret_cast<unsigned short *>(fr + nf * KK * 64);
   unsigned short *Pl = Ph + R * pitch;
-  int *oxs = reinterpret_cast<int *>(Ph + nd * R * pitch);  // [R] inverse column shifts
 
   const uint4 *gf = frags + J.frag_x;
   for (int i = threadIdx.x; i < nf * KK * 64; i += blockDim.x) fr[i] = gf[i];
-  if (threadIdx.x < R) oxs[threadIdx.x] = shifts[J.shifts + 2 * R + threadIdx.x];
   // the halo columns stay zero for every row of the block
   const int per_row = pitch - W;
   for (int i = threadIdx.x; i < R * per_row; i += blockDim.x) {
@@ -184,8 +189,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
       for (int i = 0; i < 16; ++i) {
         const int r = 8 * (i >> 2) + ((lane >> 5) << 2) + (i & 3);
         const float v = TERMS >= 2 ? c0[i] + c1[i] : c0[i];
-        const int xs = xo - oxs[r < R ? r : 0];
-        if (r < R && xo < W && xs >= 0 && xs < W) T[r * tplane + xs] = v;
+        if (r < R && xo < W) T[r * tplane + xo] = v;
       }
     }
     __syncthreads();  // P is rewritten by the next row
@@ -194,76 +198,95 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
 
 // ---------------------------------------------------- column pass (y filter)
 // One block per (message, slice r, strip of 64 columns), walking down the
-// strip in tiles of 64 output rows.  The row pass stored slice r shifted by
-// the inverse column offset, so the strip's columns are the outputs' own.
-// The source rows a tile needs (32 + 16 KK: the wave's 32 rows plus the
-// band), scaled by the slice's power of two and split into bf16 hi / lo, sit
-// in an LDS ring with one column per run of rows (the GEMM operand reads 8
+// strip in tiles of 64 output rows.  The source rows a tile needs (32 + 16 KK:
+// a wave's 32 rows plus the band), taken at the columns x + ox the inverse
+// warp reads, scaled by the column's power of two and split into bf16 hi / lo,
+// sit in an LDS ring with one column per run of rows (the GEMM operand reads 8
 // rows of one column per lane; staging writes 16 rows of one column per
 // thread); the next tile reuses all but 64 of them.  Each wave computes a
-// 32 x 32 quarter of the tile, then the epilogue of the FP32 column pass with
-// its operands loaded before the GEMM.
-template <int TERMS>
-__global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
-                                                      const uint4 *__restrict__ frags, const int *__restrict__ shifts,
-                                                      int R, int xtiles) {
-  extern __shared__ uint4 lds[];
-  const MsgJob &J = jobs[blockIdx.y];
+// 32 x 32 quarter of the tile, then the caller's accumulation.
+//
+// The accumulation is a template (EPI), so that every load of a tile is
+// unconditional straight-line code: per tile a wave issues its epilogue
+// operands, then the next tile's ring rows (always 16 loads, clamped into the
+// grid), and the epilogue waits for its operands only while the ring loads
+// stay in flight through it.
+enum : int {
+  kEpiUpRoot = 0,  // message slot of the root: msg
+  kEpiUpNode = 1,  // chain-node belief: msg
+  kEpiUpNodeU = 2, // chain-node belief: msg + masked unary
+  kEpiDownLeaf = 3, // marginal of a leaf: (masked unary or 0) + msg
+  kEpiDownMid = 4,  // marginal: stored belief + msg; next downward source: masked unary + msg
+};
+
+template <int TERMS, int EPI>
+__device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
+                                                   const int *__restrict__ shifts, int R, int r, int xt,
+                                                   uint4 *lds) {
+  constexpr bool kNeedU = EPI >= kEpiUpNodeU;  // unary of the part written
+  constexpr bool kNeedB = EPI == kEpiDownMid;  // stored upward belief
+  constexpr bool kDown = EPI >= kEpiDownLeaf;
   const int H = J.H, W = J.W;
-  const int r = blockIdx.x / xtiles;
-  const int xt = blockIdx.x - r * xtiles;
-  if (xt * 64 >= W) return;
   const int KK = J.kk_y;
+  const int ky = J.ky;
+  const int tp = J.tpitch;
   const int nrows = mfma_ring_rows(KK);
   const int cs = mfma_ring_pitch(KK);
-  const int nf = TERMS == 3 ? 2 : 1;  // tap parts
-  uint4 *fr = lds;  // [nf][KK][64] Toeplitz fragments
+  constexpr int nf = TERMS == 3 ? 2 : 1;  // tap parts
+  uint4 *fr = lds;                        // [nf][KK][64] Toeplitz fragments
   unsigned short *Rh = reinterpret_cast<unsigned short *>(lds + nf * KK * 64);
   unsigned short *Rl = Rh + 64 * cs;
+  float *cexp = reinterpret_cast<float *>(Rl + (TERMS >= 2 ? 64 * cs : 0));  // [64] exponent of each column
+  unsigned *cmax = reinterpret_cast<unsigned *>(cexp + 64);                  // [64] max of the first rows
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint4 *gf = frags + J.frag_y;
   for (int i = threadIdx.x; i < nf * KK * 64; i += blockDim.x) fr[i] = gf[i];
-  const int *sh = shifts + J.shifts;
-  const int ox = sh[2 * R + r];
-  const int oy = sh[3 * R + r];
-  const int ky = J.ky;
-  const int HW = H * W;
-  float *cscale = reinterpret_cast<float *>(Rl + (TERMS >= 2 ? 64 * cs : 0));  // [64] 2^e of each column
-  unsigned *cmax = reinterpret_cast<unsigned *>(cscale + 64);                   // [64] max of the first rows
   if (threadIdx.x < 64) cmax[threadIdx.x] = 0u;
+  const int ox = shifts[J.shifts + 2 * R + r];
+  const int oy = shifts[J.shifts + 3 * R + r];
   const float M = key2f(b.max_keys[J.src_key]);
+  const int HW = H * W;
   float *dst = mptr(b, J.dst) + r * HW;
-  const float *dun = (J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : cptr(b, J.dst)) + r * HW;
-  float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
-  const bool need_u = (J.dst_kind == kDstUpNode && J.dst_add_unary) || J.dst_kind == kDstDown;
-  const bool need_b = J.dst_kind == kDstDown && J.dst_base == 0;
+  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
+  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * HW : dst;
+  const bool leaf_u = J.dst_base == 1;  // kEpiDownLeaf: a detected part (else its base is 0)
+  const UnaryMask dm = J.dst_mask;
 
-  // staging: this thread loads column xt * 64 + lane of the shifted slice, 16
-  // consecutive rows at a time (rows outside the grid are zero)
-  const int lcol = xt * 64 + lane;
-  const int tp = J.tpitch;
-  const float *Tc = cptr(b, J.tmp) + (long long)r * H * tp + (lcol < W ? lcol : 0);
-  const bool lcol_ok = lcol < W;
+  // staging: this thread loads column xt * 64 + lane + ox of the row-pass
+  // output, 16 consecutive rows at a time.  The loads go to the nearest row
+  // and column inside the grid; rows and columns outside it are zeroed where
+  // the values are consumed (mask16), so no load sits under a branch.
+  const int scol = xt * 64 + lane;
+  const int lcol = scol + ox;
+  const bool lcol_ok = scol < W && lcol >= 0 && lcol < W;
+  const float *Tc = cptr(b, J.tmp) + (long long)r * H * tp + min(max(lcol, 0), W - 1);
   float scale = 1.0f;
-  auto load16 = [&](int row0, float (&v)[16]) {
-    const bool all_in = lcol_ok && row0 >= 0 && row0 + 16 <= H;
+  auto load16 = [&](int row0, float (&v)[16]) {  // row0 is wave-uniform
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int row = row0 + k;
-      const bool in = all_in || (lcol_ok && row >= 0 && row < H);
-      const float t = Tc[(in ? row : 0) * tp];
-      v[k] = in ? t : 0.0f;
+    for (int k = 0; k < 16; ++k) v[k] = Tc[min(max(row0 + k, 0), H - 1) * tp];
+  };
+  auto mask16 = [&](int row0, float (&v)[16]) {
+    if (!lcol_ok) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = 0.0f;
+    }
+    if (row0 < 0 || row0 + 16 > H) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = (row0 + k >= 0 && row0 + k < H) ? v[k] : 0.0f;
     }
   };
-  auto put16 = [&](int prow, const float (&v)[16]) {
+  // a 16-row chunk masked, scaled and split into bf16 hi / lo, then its LDS store
+  auto split16 = [&](int row0, float (&v)[16], bf16x8_t &h0, bf16x8_t &l0, bf16x8_t &h1, bf16x8_t &l1) {
+    mask16(row0, v);
     float w[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) w[k] = v[k] * scale;
-    bf16x8_t h0, l0, h1, l1;
     split8(*reinterpret_cast<const float(*)[8]>(&w[0]), h0, l0);
     split8(*reinterpret_cast<const float(*)[8]>(&w[8]), h1, l1);
+  };
+  auto store16 = [&](int prow, const bf16x8_t &h0, const bf16x8_t &l0, const bf16x8_t &h1, const bf16x8_t &l1) {
     *reinterpret_cast<bf16x8_t *>(Rh + lane * cs + prow) = h0;
     *reinterpret_cast<bf16x8_t *>(Rh + lane * cs + prow + 8) = h1;
     if (TERMS >= 2) {
@@ -281,8 +304,13 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
 #pragma unroll
   for (int q = 0; q < kFill; ++q) {
     const int c = wave + 4 * q;
+    if (c < nrows / 16) load16(oy - ky + 16 * c, fill[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < kFill; ++q) {
+    const int c = wave + 4 * q;
     if (c < nrows / 16) {
-      load16(oy - ky + 16 * c, fill[q]);
+      mask16(oy - ky + 16 * c, fill[q]);
 #pragma unroll
       for (int k = 0; k < 16; ++k) mx = fmaxf(mx, fill[q][k]);
     }
@@ -294,42 +322,69 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
     float sc, unsc;
     slice_scale(__uint_as_float(cmax[lane]), sc, unsc);
     scale = sc;
-    if (wave == 0) cscale[lane] = unsc;
+    if (wave == 0) cexp[lane] = log2f(unsc);  // an integer in [-60, 60]
   }
 #pragma unroll
   for (int q = 0; q < kFill; ++q) {
     const int c = wave + 4 * q;
-    if (c < nrows / 16) put16(16 * c, fill[q]);
+    if (c < nrows / 16) {
+      bf16x8_t h0, l0, h1, l1;
+      split16(oy - ky + 16 * c, fill[q], h0, l0, h1, l1);
+      store16(16 * c, h0, l0, h1, l1);
+    }
   }
   __syncthreads();
 
   const int cx = (wave & 1) * 32, ry = (wave >> 1) * 32;
-  const int n = lane & 31;
-  const int kq = (lane >> 5) << 3;
+  const int n = lane & 31, h = lane >> 5;
+  const int kq = h << 3;
   const int x = xt * 64 + cx + n;
   const bool xin = x < W;
   const bool col_ok = xin && x + ox >= 0 && x + ox < W;
-  const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
+  const bool xmask = ((dm.rot_bits >> r) & 1ull) || x < dm.strip || x >= W - dm.strip;
   const unsigned short *bh_col = Rh + (cx + n) * cs;
   const unsigned short *bl_col = Rl + (cx + n) * cs;
-  const float unscale = cscale[cx + n];
-  const int xo = xin ? x : 0;
-  DstAcc da;
+  // log(v 2^e) + M = log2(v) ln 2 + (M + e ln 2)
+  constexpr float kLn2 = 0.693147180559945309f;
+  const float cM = fmaf(cexp[cx + n], kLn2, M);
+  const int xo = xin ? x : W - 1;
+  const unsigned Wb = (unsigned)W * 4u;
+
+  // reductions: max of the written values (chain-node belief, next downward
+  // source); argmax key of the marginal
+  float vmax = -INFINITY;
+  unsigned best_key = 0u, best_eb = 0u;
   int base = 0;
   for (int ty0 = 0; ty0 < H; ty0 += 64) {
-    const bool more = ty0 + 64 < H;
-    float pre[16];
-    if (more) load16(ty0 + 64 + oy - ky + nrows - 64 + 16 * wave, pre);
-    // epilogue operands of this lane's 16 outputs (rows y0 + 8 (i / 4) + i % 4)
-    const int y0 = ty0 + ry + ((lane >> 5) << 2);
+    // epilogue operands of this lane's 16 outputs, rows yl + 8 (i / 4) + i % 4,
+    // at 32-bit byte offsets from the slice (rows past the grid's end, in the
+    // last tile, read its last row)
+    const int yl = ty0 + ry + 4 * h;
+    const unsigned eb0 = (unsigned)(yl * W + xo) * 4u;
     float pu[16], pb[16];
+    if (kNeedU || kNeedB) {
+      if (ty0 + 64 <= H) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int y = y0 + 8 * (i >> 2) + (i & 3);
-      const int e = (y < H ? y : 0) * W + xo;
-      pu[i] = need_u ? dun[e] : 0.0f;
-      pb[i] = need_b ? dst[e] : 0.0f;
+        for (int i = 0; i < 16; ++i) {
+          const unsigned eb = eb0 + (unsigned)(8 * (i >> 2) + (i & 3)) * Wb;
+          if (kNeedU) pu[i] = at(dun, eb);
+          if (kNeedB) pb[i] = at(dst, eb);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const unsigned eb = (unsigned)(min(yl + 8 * (i >> 2) + (i & 3), H - 1) * W + xo) * 4u;
+          if (kNeedU) pu[i] = at(dun, eb);
+          if (kNeedB) pb[i] = at(dst, eb);
+        }
+      }
     }
+    // the next tile's new ring rows (in the last tile: clamped rows, unused)
+    const bool more = ty0 + 64 < H;
+    const int pre_row = ty0 + oy - ky + nrows + 16 * wave;
+    float pre[16];
+    load16(pre_row, pre);
+
     f32x16_t c0 = {}, c1 = {};
     int lrow = base + ry + kq;
     if (lrow >= nrows) lrow -= nrows;
@@ -342,34 +397,99 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
       lrow += 16;
       if (lrow >= nrows) lrow -= nrows;
     }
+    // epilogue values (waits for this tile's operands only)
+    float o1[16], o2[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
-      const int y = y0 + 8 * (i >> 2) + (i & 3);
-      if (!xin || y >= H) continue;
-      const int ysrc = y + oy;
+      const int dy = 8 * (i >> 2) + (i & 3);
+      const int y = yl + dy;
       const float v = TERMS >= 2 ? c0[i] + c1[i] : c0[i];
-      const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0_fast(v * unscale) + M : kLogZero;
-      dst_store(J, dst, dst2 ? dst2 + r * HW : nullptr, y * W + x, r * HW + y * W + x, xmask, msg, pu[i], pb[i],
-                da);
+      const float lg = __builtin_amdgcn_logf(v);  // -inf for 0
+      const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
+      const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
+      const bool st = xin && y < H;
+      if (EPI == kEpiUpRoot || EPI == kEpiUpNode) {
+        o1[i] = msg;
+        if (EPI == kEpiUpNode && st) vmax = fmaxf(vmax, msg);
+      } else if (EPI == kEpiUpNodeU) {
+        o1[i] = msg + (xmask ? kLogZero : pu[i]);
+        if (st) vmax = fmaxf(vmax, o1[i]);
+      } else {
+        const float u = xmask ? kLogZero : pu[i];
+        o1[i] = (kNeedB ? pb[i] : (leaf_u ? u : 0.0f)) + msg;
+        const unsigned k = f2key(o1[i]);
+        if (st && k > best_key) {  // rows ascend: the first maximum is kept
+          best_key = k;
+          best_eb = eb0 + (unsigned)dy * Wb;
+        }
+        if (kNeedB) {
+          o2[i] = u + msg;
+          if (st) vmax = fmaxf(vmax, o2[i]);
+        }
+      }
+    }
+    // the next tile's ring rows, split, before any store of this tile is
+    // issued (waiting for them then waits for nothing else)
+    bf16x8_t nh0, nl0, nh1, nl1;
+    if (more) split16(pre_row, pre, nh0, nl0, nh1, nl1);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int dy = 8 * (i >> 2) + (i & 3);
+      if (xin && yl + dy < H) {
+        const unsigned eb = eb0 + (unsigned)dy * Wb;
+        at(dst, eb) = o1[i];
+        if (kNeedB) at(dst2, eb) = o2[i];
+      }
     }
     if (more) {
       __syncthreads();  // every wave is done with this tile's ring rows
       int prow = base + 16 * wave;
       if (prow >= nrows) prow -= nrows;
-      put16(prow, pre);
+      store16(prow, nh0, nl0, nh1, nl1);
       base += 64;
       if (base >= nrows) base -= nrows;
       __syncthreads();
     }
   }
-  dst_flush(b, J, dst2 != nullptr, da, lane);
+  if (EPI == kEpiUpNode || EPI == kEpiUpNodeU || EPI == kEpiDownMid) {
+    const unsigned k = wave_max_u32(vmax > -INFINITY ? f2key(vmax) : 0u);
+    if (lane == 0 && k) atomicMax(b.max_keys + (kDown ? J.dst2_key : J.dst_key), k);
+  }
+  if (kDown) {
+    const unsigned long long m =
+        wave_max_u64(best_key ? ((unsigned long long)best_key << 32) | (0xffffffffull - (r * HW + (best_eb >> 2))) : 0ull);
+    if (lane == 0 && m) atomicMax(b.map_keys + J.map_slot, m);
+  }
+}
+
+template <int TERMS>
+__global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
+                                                      const uint4 *__restrict__ frags, const int *__restrict__ shifts,
+                                                      int R, int xtiles) {
+  extern __shared__ uint4 lds[];
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
+  const int r = (int)bx / xtiles;
+  const int xt = (int)bx - r * xtiles;
+  if (xt * 64 >= J.W) return;
+  if (J.dst_kind == kDstUpRoot)
+    colpass_mfma_strip<TERMS, kEpiUpRoot>(b, J, frags, shifts, R, r, xt, lds);
+  else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
+    colpass_mfma_strip<TERMS, kEpiUpNode>(b, J, frags, shifts, R, r, xt, lds);
+  else if (J.dst_kind == kDstUpNode)
+    colpass_mfma_strip<TERMS, kEpiUpNodeU>(b, J, frags, shifts, R, r, xt, lds);
+  else if (J.dst_base != 0)
+    colpass_mfma_strip<TERMS, kEpiDownLeaf>(b, J, frags, shifts, R, r, xt, lds);
+  else
+    colpass_mfma_strip<TERMS, kEpiDownMid>(b, J, frags, shifts, R, r, xt, lds);
 }
 
 // ------------------------------------------------------------ host side
 
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
-  return 16 * (size_t)nf * KK * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK) + 4 * (size_t)R;
+  return 16 * (size_t)nf * KK * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
 }
 size_t colpass_mfma_lds_bytes(int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;

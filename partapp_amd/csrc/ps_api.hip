@@ -221,6 +221,39 @@ struct Builder {
     return off;
   }
 
+  // Rotation filter of a message as the A operand of the row pass's second
+  // GEMM, whose B operand is the x-filter accumulator itself: B position
+  // p = 16 kk + 8 h + t (lane half h, element t) holds slice
+  // 16 kk + 8 (t / 4) + 4 h + t % 4, so lane l, step kk, element t of the
+  // table is F[l % 32][that slice] with F[ro][s] the weight of source slice s
+  // in output slice ro (circular taps, objectdetect_findrot.cpp:377-420);
+  // zero for slices >= R.  The hi block, then the lo block.
+  int add_rot_frags(const MessagePlan &mp, int R) {
+    const int off = (int)(frags.size() / 8);
+    std::vector<float> F((size_t)R * R, 0.0f);
+    if (mp.rot_mode == kRotFilter) {
+      const int LR = (int)mp.rot_taps.size();
+      const int n = (LR - 1) / 2;
+      for (int ro = 0; ro < R; ++ro)
+        for (int j = 0; j < LR; ++j) F[(size_t)ro * R + (((ro - n + j) % R) + R) % R] += mp.rot_taps[j];
+    } else if (mp.rot_mode == kRotCopy) {
+      for (int ro = 0; ro < R; ++ro) F[(size_t)ro * R + ro] = 1.0f;
+    }
+    std::vector<uint16_t> lo;
+    for (int kk = 0; kk < 2; ++kk)
+      for (int l = 0; l < 64; ++l)
+        for (int t = 0; t < 8; ++t) {
+          const int ro = l % 32, h = l / 32;
+          const int sl = 16 * kk + 8 * (t / 4) + 4 * h + t % 4;
+          const float v = (ro < R && sl < R) ? F[(size_t)ro * R + sl] : 0.0f;
+          const uint16_t hi = bf16_rne(v);
+          frags.push_back(hi);
+          lo.push_back(bf16_rne(v - bf16_float(hi)));
+        }
+    frags.insert(frags.end(), lo.begin(), lo.end());
+    return off;
+  }
+
   int add_taps(const std::vector<float> &f, int align) {
     while (taps.size() % 16) taps.push_back(0.0f);
     int off = (int)taps.size();
@@ -229,6 +262,7 @@ struct Builder {
     return off;
   }
   int add_message(const MessagePlan &mp, MsgJob &J) {
+    const int R = (int)mp.fwd_ox.size();
     J.tpitch = t_pitch(J.W);
     J.delta = mp.delta;
     J.rot_mode = mp.rot_mode;
@@ -263,6 +297,7 @@ struct Builder {
       if (J.kk_x <= kMfmaMaxKK && J.kk_y <= kMfmaMaxKK) {
         J.frag_x = add_frags(mp.fx, J.kk_x);
         J.frag_y = add_frags(mp.fy, J.kk_y);
+        J.frag_r = R <= 24 ? add_rot_frags(mp, R) : 0;
       } else {
         J.kk_x = J.kk_y = 0;  // too wide for the BF16 passes: FP32 only
       }
@@ -274,7 +309,6 @@ struct Builder {
     shifts.insert(shifts.end(), mp.inv_oy.begin(), mp.inv_oy.end());
     // the forward offsets again, indexed by output slice (source = output - delta);
     // an output slice without a source gets a row offset no grid can satisfy
-    const int R = (int)mp.fwd_ox.size();
     for (int ro = 0; ro < R; ++ro) {
       const int rs = ro - mp.delta;
       shifts.push_back(rs >= 0 && rs < R ? mp.fwd_ox[rs] : 0);
@@ -282,6 +316,16 @@ struct Builder {
     for (int ro = 0; ro < R; ++ro) {
       const int rs = ro - mp.delta;
       shifts.push_back(rs >= 0 && rs < R ? mp.fwd_oy[rs] : kNoSourceRow);
+    }
+    // the same pair packed into one word (x offset in the low, y offset in the
+    // high 16 bits, clamped to +-32767, which no grid of the BF16 passes can
+    // reach; no source: y offset -32768)
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = ro - mp.delta;
+      const bool has = rs >= 0 && rs < R;
+      const int ox = has ? std::max(-32767, std::min(32767, mp.fwd_ox[rs])) : 0;
+      const int oy = has ? std::max(-32767, std::min(32767, mp.fwd_oy[rs])) : -32768;
+      shifts.push_back((int)(((unsigned)oy << 16) | ((unsigned)ox & 0xffffu)));
     }
     return PS_OK;
   }
@@ -540,7 +584,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         } else {
           ++L.n_diag;
           L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
-          if (j.kk_x < 1 || j.kk_y < 1 || j.W > 512) L.mfma_ok = false;
+          // (the BF16 passes address a row-pass output grid with 32-bit byte offsets)
+          if (j.kk_x < 1 || j.kk_y < 1 || j.W > 512 || j.H > 32767 || 4ll * R * j.H * j.tpitch >= (1ll << 32))
+            L.mfma_ok = false;
           for (int t = 0; t < 3 && L.mfma_ok; ++t) {
             L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kx, j.kk_x, t + 1));
             L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t + 1));

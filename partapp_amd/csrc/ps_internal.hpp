@@ -58,7 +58,7 @@ struct MsgJob {
   int taps_x;    //   with zeros to a multiple of 16
   int taps_y;
   int shifts;    // offset into the shift pool: fwd_ox[R] fwd_oy[R] inv_ox[R] inv_oy[R] (by source
-                 // slice), then fwd_ox[R] fwd_oy[R] by output slice
+                 // slice), then fwd_ox[R] fwd_oy[R] by output slice, then those packed in 16 bits each
   int src_kind;  // SrcKind
   int src_key;   // index of the source's max key
   GridRef src;   // [R][H][W]
@@ -80,7 +80,7 @@ struct MsgJob {
   int kk_x, kk_y;
   int frag_x, frag_y;
   int tpitch;  // row pitch (floats) of the row-pass output grid, a multiple of 32
-  int pad2_;
+  int frag_r;  // rotation filter as the A operand of the row pass's second GEMM
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the

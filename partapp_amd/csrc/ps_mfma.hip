@@ -92,19 +92,30 @@ __host__ __device__ __forceinline__ int mfma_ring_rows(int KK) { return 32 + 16 
 __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ring_rows(KK) + 8; }
 
 // ------------------------------------------------------- row pass (x filter)
-// One block per (message, run of `rows` output rows), one lane per column
-// (W <= blockDim.x).  For each row y: the lane's column of all R slices
-// (shift, exp, rotation filter) is stored split into bf16 hi / lo rows
-// P[r][kx + x] in LDS, while the loads of the next row are already in
-// flight; then each wave takes 32-wide column tiles and runs the KK-step GEMM
-// P (R x K, rows clamped to R - 1 beyond R) times the Toeplitz fragments, one
-// accumulator for hi*hi and one for the two cross terms.  The row pass works
-// on exp(b - M) as the FP32 pass does (M the source grid's maximum).
+// One block per (message, run of `rows` output rows); wave w gathers columns
+// 64 w .. 64 w + 63 and computes the two 32-wide output tiles over them.  For
+// each row y: the lane's column of all R slices (forward shift, masks,
+// exp(b - M), with M the source grid's maximum) is stored as bf16 (hi, and lo
+// for TERMS >= 2) in LDS rows P[slice][kx + x]; then each wave runs the x
+// filter as the KK-step GEMM P (R x K, rows clamped to R - 1 beyond R) times
+// the Toeplitz fragments, and the rotation filter as a second GEMM F (R x R,
+// circular taps) times that result, taken straight from the accumulator
+// registers: lane l holds rows 8 (i / 4) + 4 (l / 32) + i % 4 of column l % 32
+// in accumulator i, which is the B operand of k-step i / 8 once the K index
+// is renumbered, so the host stores F with its columns permuted accordingly.
+// Both filters are linear and act on different axes, so the order x filter
+// then rotation equals the reference's rotation then x filter.  The rotation
+// GEMM splits its data operand into bf16 hi + lo (and the taps too when
+// TERMS = 3).
+//
+// The loads of row y + 1 are issued before row y's GEMMs and stay in flight
+// through them; every load is unconditional (addresses clamped into the
+// grid) and every slice's masking is branch-free.
 template <int R, int TERMS>
 __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
-                                                      const float *__restrict__ taps, const int *__restrict__ shifts,
-                                                      const uint4 *__restrict__ frags, int rows) {
-  static_assert(R <= 32, "one 32-row MFMA tile holds every rotation slice");
+                                                      const int *__restrict__ shifts, const uint4 *__restrict__ frags,
+                                                      int rows) {
+  static_assert(R <= 24, "the rotation GEMM keeps output slices 0..23");
   extern __shared__ uint4 lds[];
   unsigned bx, by;
   xcd_block(bx, by);
@@ -114,15 +125,19 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   if (y0 >= H) return;
   const int y1 = min(H, y0 + rows);
   const int kx = J.kx, KK = J.kk_x;
-  const int nf = TERMS == 3 ? 2 : 1;  // tap parts
-  const int nd = TERMS >= 2 ? 2 : 1;  // data parts
+  constexpr int nf = TERMS == 3 ? 2 : 1;  // x tap parts
+  constexpr int nd = TERMS >= 2 ? 2 : 1;  // data parts
+  constexpr int nr = TERMS == 3 ? 2 : 1;  // rotation tap parts
   const int pitch = mfma_row_pitch(W, kx, KK);
-  uint4 *fr = lds;  // [nf][KK][64] Toeplitz fragments
-  unsigned short *Ph = reinterpret_cast<unsigned short *>(fr + nf * KK * 64);
+  uint4 *fr = lds;                 // [nf][KK][64] Toeplitz fragments
+  uint4 *frr = fr + nf * KK * 64;  // [nr][2][64] rotation fragments
+  unsigned short *Ph = reinterpret_cast<unsigned short *>(frr + nr * 2 * 64);
   unsigned short *Pl = Ph + R * pitch;
 
   const uint4 *gf = frags + J.frag_x;
   for (int i = threadIdx.x; i < nf * KK * 64; i += blockDim.x) fr[i] = gf[i];
+  const uint4 *gr = frags + J.frag_r;
+  for (int i = threadIdx.x; i < nr * 2 * 64; i += blockDim.x) frr[i] = gr[i];
   // the halo columns stay zero for every row of the block
   const int per_row = pitch - W;
   for (int i = threadIdx.x; i < R * per_row; i += blockDim.x) {
@@ -133,65 +148,158 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
     if (TERMS >= 2) Pl[col] = 0;
   }
 
+  // exp(b - M) = 2^(b log2 e - M log2 e)
+  constexpr float kLog2e = 1.4426950408889634f;
   const float M = key2f(b.max_keys[J.src_key]);
-  const float negM = -M;
-  const float e_fill = expf(kLogZero + negM);
+  const float mM = -M * kLog2e;
   const float *src = cptr(b, J.src);
-  const int *oxo = shifts + J.shifts + 4 * R;
-  const int *oyo = oxo + R;
-  const float *fr_rot = taps + J.taps_r;
   const bool zero_src = J.src_kind == kSrcZero;
   const UnaryMask mk = J.src_mask;
   const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
-  const int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
-  const int tp = J.tpitch;
-  const int tplane = H * tp;
-  const int x = threadIdx.x;
-  const bool has_x = x < W;
-  const int xl = has_x ? x : 0;
+  int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
+  int delta = J.delta;
+  // forward offsets by output slice (the source slice is ro - delta), packed
+  // x | y << 16; read through the scalar cache per row (the offset is opaque
+  // to the loop optimiser so that they do not occupy scalar registers
+  // throughout)
+  int so = J.shifts + 6 * R;
+  // The lane's column is re-read through an empty asm every row, so the loop
+  // optimiser cannot hoist the per-slice addresses derived from it (2 R of
+  // them) out of the row loop and hold them in registers for its lifetime.
+  auto lane_column = [] {
+    int c = threadIdx.x;
+    asm volatile("" : "+v"(c));
+    return c;
+  };
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // raw source values of row y (addresses clamped into the grid: every load
+  // is unconditional) and their masking when consumed; both read the 2 R
+  // offsets in one batch of scalar loads and are branch-free per slice
+  auto offsets = [&](int (&ox)[R], int (&oy)[R]) {
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) {
+      const int pk = shifts[so + ro];
+      ox[ro] = (int)(short)(pk & 0xffff);
+      oy[ro] = pk >> 16;
+    }
+  };
+  auto gather = [&](int y, float (&v)[R]) {
+    int ox[R], oy[R];
+    offsets(ox, oy);
+    const int xl = min(lane_column(), W - 1);
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) {
+      const int rs = min(max(ro - delta, 0), R - 1);
+      const int yy = min(max(y + oy[ro], 0), H - 1);
+      const int xx = min(max(xl + ox[ro], 0), W - 1);
+      v[ro] = src[rs * HW + yy * W + xx];
+    }
+  };
+  auto consume = [&](int y, const float (&v)[R]) {
+    int ox[R], oy[R];
+    offsets(ox, oy);
+    const int x = lane_column();
+    float e[R];
+    if (zero_src) {
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) e[ro] = 0.0f;
+    } else if (use_mask) {
+      // no source pixel, or a masked unary state: LOG_ZERO (objectdetect_findrot.cpp:342-365)
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) {
+        const int xx = x + ox[ro];
+        const int rs = ro - delta;
+        const bool rot_masked = (unsigned)rs < (unsigned)R && ((mk.rot_bits >> (rs & 63)) & 1ull);
+        const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)xx < (unsigned)W && !rot_masked &&
+                        xx >= mk.strip && xx < W - mk.strip;
+        e[ro] = __builtin_amdgcn_exp2f(fmaf(ok ? v[ro] : kLogZero, kLog2e, mM));
+      }
+    } else {
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) {
+        const int xx = x + ox[ro];
+        const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)xx < (unsigned)W;
+        e[ro] = __builtin_amdgcn_exp2f(fmaf(ok ? v[ro] : kLogZero, kLog2e, mM));
+      }
+    }
+    if (x < W) {
+#pragma unroll
+      for (int ro = 0; ro < R; ++ro) {
+        const __bf16 hi = (__bf16)e[ro];
+        Ph[ro * pitch + kx + x] = __builtin_bit_cast(unsigned short, hi);
+        if (TERMS >= 2) Pl[ro * pitch + kx + x] = bf16_bits(e[ro] - (float)hi);
+      }
+    }
+  };
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = min(lane & 31, R - 1);
-  const int kq = (lane >> 5) << 3;
+  const int h = lane >> 5;
+  const int kq = h << 3;
   const int ntile = (W + 31) >> 5;
+  const int t0 = min(2 * wave, ntile - 1), t1 = min(2 * wave + 1, ntile - 1);
+  // byte offsets of this lane's outputs: slice 8 (i / 4) + 4 h + i % 4, column
+  // 32 t + lane % 32, from the row's start
+  const int tp = J.tpitch;
+  unsigned slice_b = (unsigned)H * (unsigned)tp * 4u;
+  const unsigned ob0 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t0 + (lane & 31)) * 4u;
+  const unsigned ob1 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t1 + (lane & 31)) * 4u;
+  float *Tbase = mptr(b, J.tmp);
 
-  // the loads of row y + 1 are issued once row y is in LDS, and are in flight
-  // during row y's GEMM
+  // x filter and rotation of one 32-column tile; res[i] is output slice
+  // 8 (i / 4) + 4 h + i % 4 (< 24) of column 32 t + lane % 32
+  auto tile = [&](int t, float (&res)[12]) {
+    const unsigned short *ah = Ph + m * pitch + (t << 5) + kq;
+    const unsigned short *al = Pl + m * pitch + (t << 5) + kq;
+    f32x16_t c = {};
+    for (int kk = 0; kk < KK; ++kk) {
+      const bf16x8_t a_hi = ld_frag(ah + 16 * kk);
+      const bf16x8_t b_hi = ld_frag(fr + kk * 64 + lane);
+      c = mfma(a_hi, b_hi, c);
+      if (TERMS >= 2) c = mfma(ld_frag(al + 16 * kk), b_hi, c);
+      if (TERMS == 3) c = mfma(a_hi, ld_frag(fr + (KK + kk) * 64 + lane), c);
+    }
+    // rotation: k-step s takes accumulators 8 s .. 8 s + 7, split hi + lo
+    f32x16_t d = {};
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      float w[8];
+#pragma unroll
+      for (int t8 = 0; t8 < 8; ++t8) w[t8] = c[8 * st + t8];
+      bf16x8_t bh, bl;
+      split8(w, bh, bl);
+      const bf16x8_t f = ld_frag(frr + 64 * st + lane);
+      d = mfma(f, bh, d);
+      d = mfma(f, bl, d);
+      if (TERMS == 3) d = mfma(ld_frag(frr + 128 + 64 * st + lane), bh, d);
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) res[i] = d[i];
+  };
+  auto store_tile = [&](unsigned ob, float *Trow, const float (&res)[12]) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) at(Trow, ob + (unsigned)(8 * (i >> 2) + (i & 3)) * slice_b) = res[i];
+  };
+
+  // Row y: its values into LDS, the loads of row y + 1 issued (in flight
+  // through the GEMMs), then both tiles and their stores.
   float v[R];
-  unsigned long long ok = row_loads<R>(src, oxo, oyo, H, W, HW, J.delta, zero_src, y0, xl, v);
+  gather(y0, v);
   for (int y = y0; y < y1; ++y) {
-    if (has_x) {
-      float o[R];
-      row_finish<R, true>(v, ok, oxo, fr_rot, negM, e_fill, W, J.delta, J.rot_mode, J.rot_len, use_mask, mk, x, o);
-#pragma unroll
-      for (int ro = 0; ro < R; ++ro) Ph[ro * pitch + kx + x] = bf16_bits(o[ro]);
-      if (TERMS >= 2) {
-#pragma unroll
-        for (int ro = 0; ro < R; ++ro) Pl[ro * pitch + kx + x] = bf16_bits(o[ro] - (float)(__bf16)o[ro]);
-      }
-    }
-    if (y + 1 < y1) ok = row_loads<R>(src, oxo, oyo, H, W, HW, J.delta, zero_src, y + 1, xl, v);
-    __syncthreads();
-    float *T = mptr(b, J.tmp) + y * tp;
-    for (int t = wave; t < ntile; t += nw) {
-      const unsigned short *ah = Ph + m * pitch + (t << 5) + kq;
-      const unsigned short *al = Pl + m * pitch + (t << 5) + kq;
-      f32x16_t c0 = {}, c1 = {};
-      for (int kk = 0; kk < KK; ++kk) {
-        const bf16x8_t a_hi = ld_frag(ah + 16 * kk);
-        const bf16x8_t b_hi = ld_frag(fr + kk * 64 + lane);
-        c0 = mfma(a_hi, b_hi, c0);
-        if (TERMS >= 2) c1 = mfma(ld_frag(al + 16 * kk), b_hi, c1);
-        if (TERMS == 3) c1 = mfma(a_hi, ld_frag(fr + (KK + kk) * 64 + lane), c1);
-      }
-      const int xo = (t << 5) + (lane & 31);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 8 * (i >> 2) + ((lane >> 5) << 2) + (i & 3);
-        const float v = TERMS >= 2 ? c0[i] + c1[i] : c0[i];
-        if (r < R && xo < W) T[r * tplane + xo] = v;
-      }
-    }
+    // per-slice scalars are recomputed every row rather than hoisted
+    asm volatile("" : "+s"(so), "+s"(HW), "+s"(delta), "+s"(slice_b));
+    consume(y, v);
+    __builtin_amdgcn_sched_barrier(0);
+    gather(min(y + 1, y1 - 1), v);
+    __syncthreads();  // row y is in LDS
+    float *Trow = Tbase + (long long)y * tp;
+    float res[12];
+    tile(t0, res);
+    store_tile(ob0, Trow, res);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(t1, res);
+    store_tile(ob1, Trow, res);
     __syncthreads();  // P is rewritten by the next row
   }
 }
@@ -489,7 +597,7 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
 
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
-  return 16 * (size_t)nf * KK * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
+  return 16 * (size_t)nf * (KK + 2) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
 }
 size_t colpass_mfma_lds_bytes(int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
@@ -519,11 +627,11 @@ hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
   dim3 grid((max_h + rows - 1) / rows, n_jobs);
   const int nt = std::min(512, std::max(64, (max_w + 63) / 64 * 64));
   if (terms == 3)
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 3>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts, frags, rows);
+    hipLaunchKernelGGL((k_rowpass_mfma<24, 3>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   else if (terms == 2)
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 2>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts, frags, rows);
+    hipLaunchKernelGGL((k_rowpass_mfma<24, 2>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   else
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 1>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts, frags, rows);
+    hipLaunchKernelGGL((k_rowpass_mfma<24, 1>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   return hipGetLastError();
 }
 

@@ -337,7 +337,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   const int H = J.H, W = J.W;
   const int KK = J.kk_y;
   const int ky = J.ky;
-  const int tp = J.tpitch;
+  int tp = J.tpitch;
   const int nrows = mfma_ring_rows(KK);
   const int cs = mfma_ring_pitch(KK);
   constexpr int nf = TERMS == 3 ? 2 : 1;  // tap parts
@@ -456,7 +456,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   constexpr float kLn2 = 0.693147180559945309f;
   const float cM = fmaf(cexp[cx + n], kLn2, M);
   const int xo = xin ? x : W - 1;
-  const unsigned Wb = (unsigned)W * 4u;
+  unsigned Wb = (unsigned)W * 4u;
 
   // reductions: max of the written values (chain-node belief, next downward
   // source); argmax key of the marginal
@@ -464,6 +464,9 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   unsigned best_key = 0u, best_eb = 0u;
   int base = 0;
   for (int ty0 = 0; ty0 < H; ty0 += 64) {
+    // the per-row offsets are recomputed every tile rather than hoisted into
+    // registers for the whole loop
+    asm volatile("" : "+s"(Wb), "+s"(tp));
     // epilogue operands of this lane's 16 outputs, rows yl + 8 (i / 4) + i % 4,
     // at 32-bit byte offsets from the slice (rows past the grid's end, in the
     // last tile, read its last row)
@@ -493,60 +496,54 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
     float pre[16];
     load16(pre_row, pre);
 
-    f32x16_t c0 = {}, c1 = {};
+    f32x16_t c = {};
     int lrow = base + ry + kq;
     if (lrow >= nrows) lrow -= nrows;
     for (int kk = 0; kk < KK; ++kk) {
       const bf16x8_t a_hi = ld_frag(fr + kk * 64 + lane);
       const bf16x8_t b_hi = ld_frag(bh_col + lrow);
-      c0 = mfma(a_hi, b_hi, c0);
-      if (TERMS >= 2) c1 = mfma(a_hi, ld_frag(bl_col + lrow), c1);
-      if (TERMS == 3) c1 = mfma(ld_frag(fr + (KK + kk) * 64 + lane), b_hi, c1);
+      c = mfma(a_hi, b_hi, c);
+      if (TERMS >= 2) c = mfma(a_hi, ld_frag(bl_col + lrow), c);
+      if (TERMS == 3) c = mfma(ld_frag(fr + (KK + kk) * 64 + lane), b_hi, c);
       lrow += 16;
       if (lrow >= nrows) lrow -= nrows;
     }
-    // epilogue values (waits for this tile's operands only)
-    float o1[16], o2[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int dy = 8 * (i >> 2) + (i & 3);
-      const int y = yl + dy;
-      const float v = TERMS >= 2 ? c0[i] + c1[i] : c0[i];
-      const float lg = __builtin_amdgcn_logf(v);  // -inf for 0
-      const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
-      const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
-      const bool st = xin && y < H;
-      if (EPI == kEpiUpRoot || EPI == kEpiUpNode) {
-        o1[i] = msg;
-        if (EPI == kEpiUpNode && st) vmax = fmaxf(vmax, msg);
-      } else if (EPI == kEpiUpNodeU) {
-        o1[i] = msg + (xmask ? kLogZero : pu[i]);
-        if (st) vmax = fmaxf(vmax, o1[i]);
-      } else {
-        const float u = xmask ? kLogZero : pu[i];
-        o1[i] = (kNeedB ? pb[i] : (leaf_u ? u : 0.0f)) + msg;
-        const unsigned k = f2key(o1[i]);
-        if (st && k > best_key) {  // rows ascend: the first maximum is kept
-          best_key = k;
-          best_eb = eb0 + (unsigned)dy * Wb;
-        }
-        if (kNeedB) {
-          o2[i] = u + msg;
-          if (st) vmax = fmaxf(vmax, o2[i]);
-        }
-      }
-    }
     // the next tile's ring rows, split, before any store of this tile is
-    // issued (waiting for them then waits for nothing else)
+    // issued (that wait covers this tile's operands as well, all issued
+    // before the GEMM)
     bf16x8_t nh0, nl0, nh1, nl1;
     if (more) split16(pre_row, pre, nh0, nl0, nh1, nl1);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int dy = 8 * (i >> 2) + (i & 3);
-      if (xin && yl + dy < H) {
+      const int y = yl + dy;
+      const float lg = __builtin_amdgcn_logf(c[i]);  // -inf for 0
+      const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
+      const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
+      if (xin && y < H) {
         const unsigned eb = eb0 + (unsigned)dy * Wb;
-        at(dst, eb) = o1[i];
-        if (kNeedB) at(dst2, eb) = o2[i];
+        if (EPI == kEpiUpRoot || EPI == kEpiUpNode) {
+          at(dst, eb) = msg;
+          if (EPI == kEpiUpNode) vmax = fmaxf(vmax, msg);
+        } else if (EPI == kEpiUpNodeU) {
+          const float vv = msg + (xmask ? kLogZero : pu[i]);
+          at(dst, eb) = vv;
+          vmax = fmaxf(vmax, vv);
+        } else {
+          const float u = xmask ? kLogZero : pu[i];
+          const float mg = (kNeedB ? pb[i] : (leaf_u ? u : 0.0f)) + msg;
+          at(dst, eb) = mg;
+          const unsigned k = f2key(mg);
+          if (k > best_key) {  // rows ascend: the first maximum is kept
+            best_key = k;
+            best_eb = eb;
+          }
+          if (kNeedB) {
+            const float v2 = u + msg;
+            at(dst2, eb) = v2;
+            vmax = fmaxf(vmax, v2);
+          }
+        }
       }
     }
     if (more) {

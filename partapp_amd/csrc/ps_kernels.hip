@@ -458,54 +458,67 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
 
 // ----------------------------------------------------------------- k_root
 
-__global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict__ jobs, int R) {
-  const RootJob &J = jobs[blockIdx.y];
+// KC: the number of root children when it is at most 8 (fully unrolled),
+// else kMaxRootChildren with the count read at run time.
+template <int KC>
+__device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, int R, int pix0) {
   const int H = J.H, W = J.W;
   const int HW = H * W;
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x * blockDim.x >= HW) return;
+  const int pix = pix0 + threadIdx.x;
   const bool ok = pix < HW;
-  const int x = ok ? pix % W : 0;
-  const int K = J.K;
+  const int pc = ok ? pix : HW - 1;  // loads of lanes past the end read the last pixel
+  const int x = pc % W;
+  const int K = KC <= 8 ? KC : J.K;
   const float *un = cptr(b, J.unary);
   float *marg = mptr(b, J.marginal);
 
-  unsigned kmax[kMaxRootChildren];
+  unsigned kmax[KC];
 #pragma unroll
-  for (int k = 0; k < kMaxRootChildren; ++k) kmax[k] = 0u;
+  for (int k = 0; k < KC; ++k) kmax[k] = 0u;
   unsigned long long amax = 0ull;
   float sum = 0.0f;
   bool first = true;
 
-  for (int r = 0; r < R; ++r) {
-    if (!ok) break;
-    const long long e = (long long)r * HW + pix;
-    float m[kMaxRootChildren];
+  // the loads of slice r + 1 are issued before slice r is processed (its
+  // stores could alias them as far as the compiler knows)
+  float m[KC], u;
+  auto load = [&](int r, float (&mm)[KC], float &uu) {
+    const int e = r * HW + pc;
 #pragma unroll
-    for (int k = 0; k < kMaxRootChildren; ++k)
-      if (k < K) m[k] = cptr(b, J.msg[k])[e];
-    const float u = masked(J.mask, r, x, W) ? kLogZero : un[e];
+    for (int k = 0; k < KC; ++k) mm[k] = k < K ? cptr(b, J.msg[k])[e] : 0.0f;
+    uu = un[e];
+  };
+  load(0, m, u);
+  for (int r = 0; r < R; ++r) {
+    float mn[KC], un_next;
+    load(r + 1 < R ? r + 1 : r, mn, un_next);
+    const int e = r * HW + pix;
+    const float uu = masked(J.mask, r, x, W) ? kLogZero : u;
     // root belief: messages in joint order (:637), then the unary (:652-654)
     float bel = 0.0f;
 #pragma unroll
-    for (int k = 0; k < kMaxRootChildren; ++k)
+    for (int k = 0; k < KC; ++k)
       if (k < K) bel = bel + m[k];
-    if (J.add_unary) bel = bel + u;
-    marg[e] = bel;
-    const unsigned long long ak = amax_key(bel, (unsigned)e);
-    amax = ak > amax ? ak : amax;
+    if (J.add_unary) bel = bel + uu;
+    if (ok) {
+      marg[e] = bel;
+      const unsigned long long ak = amax_key(bel, (unsigned)e);
+      amax = ak > amax ? ak : amax;
+    }
     // log_from_root of child i: the other messages in order (:641-649), then
     // the root unary (:169)
 #pragma unroll
-    for (int i = 0; i < kMaxRootChildren; ++i) {
+    for (int i = 0; i < KC; ++i) {
       if (i < K) {
         float fr = 0.0f;
 #pragma unroll
-        for (int k = 0; k < kMaxRootChildren; ++k)
+        for (int k = 0; k < KC; ++k)
           if (k < K && k != i) fr = fr + m[k];
-        fr = fr + u;
-        mptr(b, J.msg[i])[e] = fr;
-        kmax[i] = max(kmax[i], f2key(fr));
+        fr = fr + uu;
+        if (ok) {
+          mptr(b, J.msg[i])[e] = fr;
+          kmax[i] = max(kmax[i], f2key(fr));
+        }
       }
     }
     // root posterior over the valid rotations (:713-726), no max shift
@@ -514,6 +527,9 @@ __global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict
       sum = first ? ev : sum + ev;
       first = false;
     }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) m[k] = mn[k];
+    u = un_next;
   }
   if (ok && J.root_post.buf >= 0 && b.root) mptr(b, J.root_post)[pix] = log0(sum);
 
@@ -521,11 +537,30 @@ __global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict
   amax = wave_max_u64(amax);
   if (lane == 0 && amax) atomicMax(b.map_keys + J.map_slot, amax);
 #pragma unroll
-  for (int k = 0; k < kMaxRootChildren; ++k) {
+  for (int k = 0; k < KC; ++k) {
     if (k < K) {
       unsigned v = wave_max_u32(kmax[k]);
       if (lane == 0 && v) atomicMax(b.max_keys + J.fr_key[k], v);
     }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict__ jobs, int R) {
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const RootJob &J = jobs[by];
+  const int pix0 = (int)bx * blockDim.x;
+  if (pix0 >= J.H * J.W) return;
+  switch (J.K) {
+    case 1: root_pixels<1>(b, J, R, pix0); break;
+    case 2: root_pixels<2>(b, J, R, pix0); break;
+    case 3: root_pixels<3>(b, J, R, pix0); break;
+    case 4: root_pixels<4>(b, J, R, pix0); break;
+    case 5: root_pixels<5>(b, J, R, pix0); break;
+    case 6: root_pixels<6>(b, J, R, pix0); break;
+    case 7: root_pixels<7>(b, J, R, pix0); break;
+    case 8: root_pixels<8>(b, J, R, pix0); break;
+    default: root_pixels<kMaxRootChildren>(b, J, R, pix0); break;
   }
 }
 

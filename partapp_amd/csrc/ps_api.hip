@@ -39,7 +39,7 @@ hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
                                const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
                                size_t lds_bytes, hipStream_t s);
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
-                               int R, int terms, int max_w, size_t lds_bytes, hipStream_t s);
+                               int R, int terms, bool t16, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
 }  // namespace ps
 
@@ -298,6 +298,17 @@ struct Builder {
         J.frag_x = add_frags(mp.fx, J.kk_x);
         J.frag_y = add_frags(mp.fy, J.kk_y);
         J.frag_r = R <= 24 ? add_rot_frags(mp, R) : 0;
+        // the row-pass output is at most (sum of x taps) (largest row sum of
+        // the rotation filter) times max exp(b - M) = 1; its fp16 copy holds
+        // it scaled by 2^t16_shift, that bound (with margin) near 2^14
+        double sx = 0.0, sr = 0.0;
+        for (float f : mp.fx) sx += std::fabs(f);
+        if (mp.rot_mode == kRotFilter)
+          for (float f : mp.rot_taps) sr += std::fabs(f);
+        else if (mp.rot_mode == kRotCopy)
+          sr = 1.0;
+        const double bound = 1.01 * sx * sr;
+        J.t16_shift = bound > 0.0 ? std::max(-100, std::min(100, (int)std::floor(14.0 - std::log2(bound)))) : 0;
       } else {
         J.kk_x = J.kk_y = 0;  // too wide for the BF16 passes: FP32 only
       }
@@ -805,7 +816,7 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     // the column pass class also times the eigenbasis stages
     ev2 = profile_begin(pp, 3, s);
     if (mf)
-      e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->col_terms, L.max_w,
+      e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->col_terms, pl->terms != 3, L.max_w,
                               L.mfma_col_lds[tc], s);
     else
       e = launch_colpass(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);

@@ -111,7 +111,7 @@ __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ri
 // The loads of row y + 1 are issued before row y's GEMMs and stay in flight
 // through them; every load is unconditional (addresses clamped into the
 // grid) and every slice's masking is branch-free.
-template <int R, int TERMS>
+template <int R, int TERMS, bool T16>
 __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
                                                       const int *__restrict__ shifts, const uint4 *__restrict__ frags,
                                                       int rows) {
@@ -241,11 +241,15 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   const int t0 = min(2 * wave, ntile - 1), t1 = min(2 * wave + 1, ntile - 1);
   // byte offsets of this lane's outputs: slice 8 (i / 4) + 4 h + i % 4, column
   // 32 t + lane % 32, from the row's start
+  // T16: the output is stored as fp16 of value * 2^t16_shift (the shift
+  // keeps the largest possible value near 2^14)
+  constexpr unsigned eb = T16 ? 2u : 4u;  // bytes per element
+  const float t_scale = T16 ? ldexpf(1.0f, J.t16_shift) : 1.0f;
   const int tp = J.tpitch;
-  unsigned slice_b = (unsigned)H * (unsigned)tp * 4u;
-  const unsigned ob0 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t0 + (lane & 31)) * 4u;
-  const unsigned ob1 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t1 + (lane & 31)) * 4u;
-  float *Tbase = mptr(b, J.tmp);
+  unsigned slice_b = (unsigned)H * (unsigned)tp * eb;
+  const unsigned ob0 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t0 + (lane & 31)) * eb;
+  const unsigned ob1 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t1 + (lane & 31)) * eb;
+  char *Tbase = reinterpret_cast<char *>(mptr(b, J.tmp));
 
   // x filter and rotation of one 32-column tile; res[i] is output slice
   // 8 (i / 4) + 4 h + i % 4 (< 24) of column 32 t + lane % 32
@@ -277,9 +281,15 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
 #pragma unroll
     for (int i = 0; i < 12; ++i) res[i] = d[i];
   };
-  auto store_tile = [&](unsigned ob, float *Trow, const float (&res)[12]) {
+  auto store_tile = [&](unsigned ob, char *Trow, const float (&res)[12]) {
 #pragma unroll
-    for (int i = 0; i < 12; ++i) at(Trow, ob + (unsigned)(8 * (i >> 2) + (i & 3)) * slice_b) = res[i];
+    for (int i = 0; i < 12; ++i) {
+      char *p = Trow + ob + (unsigned)(8 * (i >> 2) + (i & 3)) * slice_b;
+      if (T16)
+        *reinterpret_cast<_Float16 *>(p) = (_Float16)(res[i] * t_scale);
+      else
+        *reinterpret_cast<float *>(p) = res[i];
+    }
   };
 
   // Row y: its values into LDS, the loads of row y + 1 issued (in flight
@@ -293,7 +303,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
     __builtin_amdgcn_sched_barrier(0);
     gather(min(y + 1, y1 - 1), v);
     __syncthreads();  // row y is in LDS
-    float *Trow = Tbase + (long long)y * tp;
+    char *Trow = Tbase + (long long)y * tp * eb;
     float res[12];
     tile(t0, res);
     store_tile(ob0, Trow, res);
@@ -327,7 +337,7 @@ enum : int {
   kEpiDownMid = 4,  // marginal: stored belief + msg; next downward source: masked unary + msg
 };
 
-template <int TERMS, int EPI>
+template <int TERMS, int EPI, bool T16>
 __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
                                                    const int *__restrict__ shifts, int R, int r, int xt,
                                                    uint4 *lds) {
@@ -369,11 +379,17 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   const int scol = xt * 64 + lane;
   const int lcol = scol + ox;
   const bool lcol_ok = scol < W && lcol >= 0 && lcol < W;
-  const float *Tc = cptr(b, J.tmp) + (long long)r * H * tp + min(max(lcol, 0), W - 1);
+  // (T16: the row pass stored fp16 of value * 2^t16_shift)
+  const long long tcol = (long long)r * H * tp + min(max(lcol, 0), W - 1);
+  const float *Tc = cptr(b, J.tmp) + tcol;
+  const _Float16 *Tc16 = reinterpret_cast<const _Float16 *>(cptr(b, J.tmp)) + tcol;
   float scale = 1.0f;
   auto load16 = [&](int row0, float (&v)[16]) {  // row0 is wave-uniform
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = Tc[min(max(row0 + k, 0), H - 1) * tp];
+    for (int k = 0; k < 16; ++k) {
+      const int row = min(max(row0 + k, 0), H - 1);
+      v[k] = T16 ? (float)Tc16[row * tp] : Tc[row * tp];
+    }
   };
   auto mask16 = [&](int row0, float (&v)[16]) {
     if (!lcol_ok) {
@@ -430,7 +446,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
     float sc, unsc;
     slice_scale(__uint_as_float(cmax[lane]), sc, unsc);
     scale = sc;
-    if (wave == 0) cexp[lane] = log2f(unsc);  // an integer in [-60, 60]
+    if (wave == 0) cexp[lane] = log2f(unsc) - (T16 ? (float)J.t16_shift : 0.0f);  // an integer
   }
 #pragma unroll
   for (int q = 0; q < kFill; ++q) {
@@ -567,7 +583,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   }
 }
 
-template <int TERMS>
+template <int TERMS, bool T16>
 __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
                                                       const uint4 *__restrict__ frags, const int *__restrict__ shifts,
                                                       int R, int xtiles) {
@@ -579,15 +595,15 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
   const int xt = (int)bx - r * xtiles;
   if (xt * 64 >= J.W) return;
   if (J.dst_kind == kDstUpRoot)
-    colpass_mfma_strip<TERMS, kEpiUpRoot>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiUpRoot, T16>(b, J, frags, shifts, R, r, xt, lds);
   else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
-    colpass_mfma_strip<TERMS, kEpiUpNode>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiUpNode, T16>(b, J, frags, shifts, R, r, xt, lds);
   else if (J.dst_kind == kDstUpNode)
-    colpass_mfma_strip<TERMS, kEpiUpNodeU>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiUpNodeU, T16>(b, J, frags, shifts, R, r, xt, lds);
   else if (J.dst_base != 0)
-    colpass_mfma_strip<TERMS, kEpiDownLeaf>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiDownLeaf, T16>(b, J, frags, shifts, R, r, xt, lds);
   else
-    colpass_mfma_strip<TERMS, kEpiDownMid>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiDownMid, T16>(b, J, frags, shifts, R, r, xt, lds);
 }
 
 // ------------------------------------------------------------ host side
@@ -606,12 +622,13 @@ hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
     if (bytes <= 64 * 1024) return hipSuccess;
     return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   };
-  hipError_t e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<1>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<2>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
+  hipError_t e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1, true>), row_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<1, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<2, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3, false>), col_bytes);
   return e;
 }
 
@@ -624,25 +641,29 @@ hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
   dim3 grid((max_h + rows - 1) / rows, n_jobs);
   const int nt = std::min(512, std::max(64, (max_w + 63) / 64 * 64));
   if (terms == 3)
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 3>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    hipLaunchKernelGGL((k_rowpass_mfma<24, 3, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   else if (terms == 2)
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 2>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    hipLaunchKernelGGL((k_rowpass_mfma<24, 2, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   else
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 1>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    hipLaunchKernelGGL((k_rowpass_mfma<24, 1, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   return hipGetLastError();
 }
 
+// t16: the row pass stored its output as fp16 (row-pass terms 1 and 2)
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
-                               int R, int terms, int max_w, size_t lds_bytes, hipStream_t s) {
+                               int R, int terms, bool t16, int max_w, size_t lds_bytes, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
+  if (terms < 1 || terms > 3 || (!t16 && terms != 3)) return hipErrorInvalidValue;
   const int xtiles = (max_w + 63) / 64;
   dim3 grid(R * xtiles, n_jobs);
-  if (terms == 3)
-    hipLaunchKernelGGL(k_colpass_mfma<3>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+  if (terms == 3 && t16)
+    hipLaunchKernelGGL((k_colpass_mfma<3, true>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+  else if (terms == 3)
+    hipLaunchKernelGGL((k_colpass_mfma<3, false>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
   else if (terms == 2)
-    hipLaunchKernelGGL(k_colpass_mfma<2>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+    hipLaunchKernelGGL((k_colpass_mfma<2, true>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
   else
-    hipLaunchKernelGGL(k_colpass_mfma<1>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+    hipLaunchKernelGGL((k_colpass_mfma<1, true>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
   return hipGetLastError();
 }
 

@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel roofline)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device copies of 256 MiB (the known-size kernel that scales the rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
@@ -95,21 +98,27 @@ def workload_flops(msgs, R):
 
 def mfma_flops(msgs, R, row_terms, col_terms):
     """Issued MFMA flops of the Toeplitz GEMMs per sweep (32x32x16 tiles, K
-    rounded up to steps of 16, R = 24 rotation rows padded to 32)."""
+    rounded up to steps of 16, R = 24 rotation rows padded to 32); the row
+    pass also runs the rotation filter as a 2-step GEMM per tile (data split
+    hi + lo, taps split too for 3 terms)."""
     tile = 2.0 * 32 * 32 * 16
     fl = 0.0
     for H, W, rl, kx, ky in msgs:
         kkx, kky = (32 + 2 * kx + 15) // 16, (32 + 2 * ky + 15) // 16
-        fl += tile * row_terms * kkx * ((W + 31) // 32) * H  # row pass: one 32-row tile per output row
+        rot = 2 * (3 if row_terms == 3 else 2)
+        fl += tile * (row_terms * kkx + rot) * ((W + 31) // 32) * H  # row pass: one 32-row tile per output row
         fl += tile * col_terms * kky * 4 * R * ((W + 63) // 64) * ((H + 63) // 64)  # column pass: 4 waves / 64x64
     return fl
 
 
-def grid_traffic(model):
-    """Grids (R*H*W floats) each kernel class must read or write per sweep and
-    instance, given the row pass / column pass split: row pass = source read
-    + row-filtered grid write; column pass = row-filtered grid read, the
-    destination write(s) and the unary / stored belief its epilogue adds."""
+def grid_traffic(model, t_bytes=4):
+    """Grids (R*H*W floats, 4 bytes per state) each kernel class must read or
+    write per sweep and instance, given the row pass / column pass split: row
+    pass = source read + row-filtered grid write; column pass = row-filtered
+    grid read, the destination write(s) and the unary / stored belief its
+    epilogue adds.  The row-filtered grid counts t_bytes / 4 of a grid (the
+    BF16 methods with a 1- or 2-term row pass keep it in fp16)."""
+    tg = t_bytes / 4.0
     P, root = model.num_parts, model.root
     parent = {j.child_idx: j.parent_idx for j in model.joints}
     children = {p: [c for c, q in parent.items() if q == p] for p in range(P)}
@@ -117,17 +126,53 @@ def grid_traffic(model):
     for c, p in parent.items():
         leaf = not children[c]
         # upward message c -> p
-        row += (0 if (leaf and not model.is_detect[c]) else 1) + 1
-        col += 1 + 1 + (1 if (p != root and model.is_detect[p]) else 0)
+        row += (0 if (leaf and not model.is_detect[c]) else 1) + tg
+        col += tg + 1 + (1 if (p != root and model.is_detect[p]) else 0)
         # downward message p -> c
-        row += 2
+        row += 1 + tg
         # (grid read, marginal write, unary read; with a grandchild also the
         # stored belief read and the next source write)
-        col += 1 + 1 + 1 + (2 if children[c] else 0)
+        col += tg + 1 + 1 + (2 if children[c] else 0)
     k = len(children[root])
     root_grids = k + 1 + 1 + k  # messages + unary read, marginal + log_from_root writes
     leaf_grids = sum(1 for c in range(P) if c != root and not children[c] and model.is_detect[c])
     return {"rowpass": row, "colpass": col, "root": root_grids, "leaf_max": leaf_grids}
+
+
+def device_copy_gbs(nbytes=1 << 30, reps=5):
+    """Bandwidth of a plain device-to-device copy (read + write bytes / s),
+    the practical HBM ceiling the sweep's kernels are compared with."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (t0.elapsed_time(t1) / 1e3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
+def measured_traffic(method, kernel):
+    """HBM bytes per launch of a kernel class measured by rocprofv3 (FETCH_SIZE
+    and WRITE_SIZE, scaled by the same counters on a copy of known size), read
+    from the summary committed under profiles/ for this workload, or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "traffic_%s_%s_32x300x300x24.txt" % (kernel, method))
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch"):
+                    return int(float(line.split()[1])), os.path.relpath(path, os.path.dirname(path) + "/..")
+    except OSError:
+        pass
+    return None, None
 
 
 def cpu_baseline_timed(ep, sizes, joints, seeds, budget_s):
@@ -214,8 +259,24 @@ def check_vs_f64(plan, model, unary, marg, maps, ep, joints, k):
     return worst, agree, total
 
 
+def calibrate_copy():
+    import torch
+    n = 64 * 1024 * 1024  # floats: 256 MiB
+    x = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                      "launches": 3}), flush=True)
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: instances shard by image across GPUs with no exchange, "
               "and this build measures the single-GPU share (use --gpus 1)", file=sys.stderr)
@@ -261,6 +322,8 @@ def main():
     maps = torch.zeros((plan.map_numel * 4,), dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
     print("inputs resident: %.2f GB" % (unary.numel() * 4 / 1e9), file=sys.stderr, flush=True)
+    copy_gbs = device_copy_gbs()
+    print("device copy: %.1f GB/s" % copy_gbs, file=sys.stderr, flush=True)
     stream = torch.cuda.Stream() if args.graph else torch.cuda.current_stream()
 
     fp32 = None
@@ -296,15 +359,23 @@ def main():
             print("  %-10s %9.3f ms total, %4d launches" % (k, t, c), file=sys.stderr)
         dom = max(("rowpass", "colpass", "root"), key=lambda k: prof[k][0])
         t_dom, n_dom = prof[dom]
-        traffic_grids = grid_traffic(model)
+        traffic_grids = grid_traffic(model, 2 if args.method in ("bf16", "bf16x2", "bf16mix") else 4)
         bytes_per_step = traffic_grids[dom] * 4.0 * states
         launches_per_step = n_dom / args.steps
         avg_ms = t_dom / n_dom
         achieved = bytes_per_step / launches_per_step / (avg_ms / 1e3) / 1e9
+        traffic, tsrc = (measured_traffic(args.method, dom) if args.config == 1 and n == 32 and args.size == 300
+                         else (None, None))
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes_per_launch": round(bytes_per_step / launches_per_step),
                     "avg_launch_ms": round(avg_ms, 4),
                     "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()}}
+        if tsrc:
+            roofline["traffic_source"] = tsrc
+        if copy_gbs:
+            roofline["copy_gbs"] = copy_gbs
+            roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
         # the whole sweep against the algorithmic minimum B_alg = 4 B * N_s * (2P + 4E)
         b_alg = 4.0 * states * (2 * 10 + 4 * 9)
         roofline["sweep_gbs"] = round(b_alg / (ms_per_step / 1e3) / 1e9, 1)

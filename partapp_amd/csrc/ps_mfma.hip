@@ -110,7 +110,9 @@ __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ri
 //
 // The loads of row y + 1 are issued before row y's GEMMs and stay in flight
 // through them; every load is unconditional (addresses clamped into the
-// grid) and every slice's masking is branch-free.
+// grid) and every slice's masking is branch-free.  Waves whose tiles lie past
+// a job's width (the block is sized for the widest job of the launch) only
+// gather and take part in the barriers.
 template <int R, int TERMS, bool T16>
 __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
                                                       const int *__restrict__ shifts, const uint4 *__restrict__ frags,
@@ -305,11 +307,15 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
     __syncthreads();  // row y is in LDS
     char *Trow = Tbase + (long long)y * tp * eb;
     float res[12];
-    tile(t0, res);
-    store_tile(ob0, Trow, res);
+    if (2 * wave < ntile) {  // waves past the grid's width (a narrower job of the launch) idle
+      tile(t0, res);
+      store_tile(ob0, Trow, res);
+    }
     __builtin_amdgcn_sched_barrier(0);
-    tile(t1, res);
-    store_tile(ob1, Trow, res);
+    if (2 * wave + 1 < ntile) {
+      tile(t1, res);
+      store_tile(ob1, Trow, res);
+    }
     __syncthreads();  // P is rewritten by the next row
   }
 }

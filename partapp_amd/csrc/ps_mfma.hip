@@ -105,8 +105,9 @@ __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ri
 // is renumbered, so the host stores F with its columns permuted accordingly.
 // Both filters are linear and act on different axes, so the order x filter
 // then rotation equals the reference's rotation then x filter.  The rotation
-// GEMM splits its data operand into bf16 hi + lo (and the taps too when
-// TERMS = 3).
+// GEMM splits both its operands into bf16 hi + lo for every TERMS: its taps
+// rounded to bf16 bias every message by up to 2^-9 in the same direction, which
+// several chained messages turn into an error of ~1e-3 on small grids.
 //
 // The loads of row y + 1 are issued before row y's GEMMs and stay in flight
 // through them; every load is unconditional (addresses clamped into the
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   const int kx = J.kx, KK = J.kk_x;
   constexpr int nf = TERMS == 3 ? 2 : 1;  // x tap parts
   constexpr int nd = TERMS >= 2 ? 2 : 1;  // data parts
-  constexpr int nr = TERMS == 3 ? 2 : 1;  // rotation tap parts
+  constexpr int nr = 2;                   // rotation tap parts (always hi + lo)
   const int pitch = mfma_row_pitch(W, kx, KK);
   uint4 *fr = lds;                 // [nf][KK][64] Toeplitz fragments
   uint4 *frr = fr + nf * KK * 64;  // [nr][2][64] rotation fragments
@@ -266,7 +267,8 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
       if (TERMS >= 2) c = mfma(ld_frag(al + 16 * kk), b_hi, c);
       if (TERMS == 3) c = mfma(a_hi, ld_frag(fr + (KK + kk) * 64 + lane), c);
     }
-    // rotation: k-step s takes accumulators 8 s .. 8 s + 7, split hi + lo
+    // rotation: k-step s takes accumulators 8 s .. 8 s + 7; data and taps
+    // split hi + lo (three products: hi*hi, hi*lo, lo*hi)
     f32x16_t d = {};
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
       const bf16x8_t f = ld_frag(frr + 64 * st + lane);
       d = mfma(f, bh, d);
       d = mfma(f, bl, d);
-      if (TERMS == 3) d = mfma(ld_frag(frr + 128 + 64 * st + lane), bh, d);
+      d = mfma(ld_frag(frr + 128 + 64 * st + lane), bh, d);
     }
 #pragma unroll
     for (int i = 0; i < 12; ++i) res[i] = d[i];
@@ -616,7 +618,7 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
 
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
-  return 16 * (size_t)nf * (KK + 2) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
+  return 16 * ((size_t)nf * KK + 4) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
 }
 size_t colpass_mfma_lds_bytes(int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;

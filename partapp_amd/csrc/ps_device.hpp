@@ -75,6 +75,67 @@ __device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : l
 // passes whose inputs carry 8-bit rounding already
 __device__ __forceinline__ float log0_fast(float v) { return v == 0.0f ? kLogZero : __logf(v); }
 
+// The caller's accumulation of a column pass specialised per destination
+// kind (objectdetect_findrot.cpp:637-654 upward, :201 and :220-222 downward),
+// so that a kernel's loads for it are unconditional straight-line code.
+enum : int {
+  kEpiUpRoot = 0,    // message slot of the root: msg
+  kEpiUpNode = 1,    // chain-node belief: msg
+  kEpiUpNodeU = 2,   // chain-node belief: msg + masked unary
+  kEpiDownLeaf = 3,  // marginal of a leaf: (masked unary or 0) + msg
+  kEpiDownMid = 4,   // marginal: stored belief + msg; next downward source: masked unary + msg
+};
+template <int EPI>
+struct Epi {
+  static constexpr bool kNeedU = EPI >= kEpiUpNodeU;  // reads the unary of the part written
+  static constexpr bool kNeedB = EPI == kEpiDownMid;  // reads the stored upward belief
+  static constexpr bool kDown = EPI >= kEpiDownLeaf;
+  float vmax = -INFINITY;  // max of the written next source / belief
+  unsigned best_key = 0u, best_eb = 0u;  // argmax of the marginal (key, byte offset in the slice)
+
+  // one output at byte offset eb of slice r (dst, dst2 point at the slice);
+  // pu / pb are only read when the kind needs them
+  __device__ __forceinline__ void store(float *dst, float *dst2, unsigned eb, float msg, float pu, float pb,
+                                        bool xmask, bool leaf_u) {
+    auto at = [](float *p, unsigned o) -> float & { return *reinterpret_cast<float *>(reinterpret_cast<char *>(p) + o); };
+    if (EPI == kEpiUpRoot || EPI == kEpiUpNode) {
+      at(dst, eb) = msg;
+      if (EPI == kEpiUpNode) vmax = fmaxf(vmax, msg);
+    } else if (EPI == kEpiUpNodeU) {
+      const float v = msg + (xmask ? kLogZero : pu);
+      at(dst, eb) = v;
+      vmax = fmaxf(vmax, v);
+    } else {
+      const float u = xmask ? kLogZero : pu;
+      const float mg = (kNeedB ? pb : (leaf_u ? u : 0.0f)) + msg;
+      at(dst, eb) = mg;
+      const unsigned k = f2key(mg);
+      if (k > best_key) {  // a lane's outputs come in ascending flat order: the first maximum stays
+        best_key = k;
+        best_eb = eb;
+      }
+      if (kNeedB) {
+        const float v2 = u + msg;
+        at(dst2, eb) = v2;
+        vmax = fmaxf(vmax, v2);
+      }
+    }
+  }
+  // wave reductions and one atomic per wave; every lane of the wave calls it
+  __device__ __forceinline__ void flush(const Bases &b, const MsgJob &J, long long slice0, int lane) {
+    if (EPI == kEpiUpNode || EPI == kEpiUpNodeU || EPI == kEpiDownMid) {
+      const unsigned k = wave_max_u32(vmax > -INFINITY ? f2key(vmax) : 0u);
+      if (lane == 0 && k) atomicMax(b.max_keys + (kDown ? J.dst2_key : J.dst_key), k);
+    }
+    if (kDown) {
+      const unsigned flat = (unsigned)(slice0 + (best_eb >> 2));
+      const unsigned long long m =
+          wave_max_u64(best_key ? ((unsigned long long)best_key << 32) | (0xffffffffull - flat) : 0ull);
+      if (lane == 0 && m) atomicMax(b.map_keys + J.map_slot, m);
+    }
+  }
+};
+
 // Reductions a message's epilogue feeds: the max of the next upward source,
 // the argmax of a marginal and the max of the next downward source.
 struct DstAcc {

@@ -324,112 +324,112 @@ __device__ __forceinline__ void col_chunk8(float (&acc)[kColRows], float (&buf)[
   }
 }
 
-__global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restrict__ jobs,
-                                                 const float *__restrict__ taps, const int *__restrict__ shifts,
-                                                 int R, int xtiles) {
-  extern __shared__ float4 lds4[];
-  float *tile = reinterpret_cast<float *>(lds4);
-  unsigned bx, by;
-  xcd_block(bx, by);
-  const MsgJob &J = jobs[by];
+// The caller's accumulation is a template (EPI, one variant per destination
+// kind), so every load of a tile is unconditional straight-line code: the
+// epilogue operands first, then the next tile's rows (always loaded, clamped
+// into the grid, zeroed when they reach LDS), and the epilogue waits for its
+// operands only.
+template <int EPI>
+__device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, const float *__restrict__ taps,
+                                              const int *__restrict__ shifts, int R, int r, int xt, float *tile) {
+  constexpr bool kNeedU = Epi<EPI>::kNeedU;
+  constexpr bool kNeedB = Epi<EPI>::kNeedB;
   const int H = J.H, W = J.W;
-  const int r = (int)bx / xtiles;
-  const int xt = (int)bx - r * xtiles;
-  if (xt * kColTileX >= W) return;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int x = xt * kColTileX + lane;
   const bool xin = x < W;
+  const int xo = xin ? x : W - 1;
 
-  const int *sh = shifts + J.shifts;
-  const int ox = sh[2 * R + r];
-  const int oy = sh[3 * R + r];
-  const int xs = x + ox;
-  const bool col_ok = xin && xs >= 0 && xs < W;
-  const long long HW = (long long)H * W;
-  const int tp = J.tpitch;
+  const int ox = shifts[J.shifts + 2 * R + r];
+  const int oy = shifts[J.shifts + 3 * R + r];
+  const bool col_ok = xin && x + ox >= 0 && x + ox < W;
+  const int HW = H * W;
+  int tp = J.tpitch;
   const float *Tr = cptr(b, J.tmp) + (long long)r * H * tp;
   const int ky = J.ky;
   const int Lp = col_padded_taps(ky);
   const int nrows = col_tile_rows(ky);
   const float *fy = taps + J.taps_y;
   const float M = key2f(b.max_keys[J.src_key]);
-  float *dst = mptr(b, J.dst);
-  const float *dun = J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : nullptr;
-  float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
+  float *dst = mptr(b, J.dst) + r * HW;
+  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
+  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * HW : dst;
+  const bool leaf_u = J.dst_base == 1;  // a leaf's marginal: detected part (else its base is 0)
   const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
-  const bool need_u = (J.dst_kind == kDstUpNode && J.dst_add_unary) || J.dst_kind == kDstDown;
-  const bool need_b = J.dst_kind == kDstDown && J.dst_base == 0;
-  // column loaded by this thread's lane slot, and whether it exists
+  // column loaded by this thread's lane slot (loads outside the grid go to its
+  // nearest row and column; those values are zeroed when they reach LDS)
   const int lcol = xt * kColTileX + lane + ox;
   const bool lcol_ok = x < W && lcol >= 0 && lcol < W;
-  const float *Tc = Tr + min(max(lcol, 0), W - 1);  // loads outside the grid go to its nearest column
-
-  DstAcc da;
+  const float *Tc = Tr + min(max(lcol, 0), W - 1);
+  auto src_raw = [&](int row) -> float { return Tc[(long long)min(max(row, 0), H - 1) * tp]; };
+  auto src_ok = [&](int row) -> bool { return lcol_ok && row >= 0 && row < H; };
 
   // Tile t holds source rows s_t + [0, nrows), s_t = 64 t + oy - ky.  Tile t+1
   // shares rows [64, nrows) of tile t (moved to [0, Lp) in LDS) and needs 64
-  // new rows, fetched into registers while tile t is filtered (clamped
-  // addresses + select: the loads are unconditional).
-  auto src_row = [&](int row) -> float {
-    const bool in = lcol_ok && row >= 0 && row < H;
-    const float v = Tc[(long long)(in ? row : min(max(row, 0), H - 1)) * tp];
-    return in ? v : 0.0f;
-  };
+  // new rows, fetched into registers while tile t is filtered.
   constexpr int kNew = kColTileY / kColWaves;  // new rows per thread
-  float pre[kNew];
-  for (int rr = wave; rr < nrows; rr += kColWaves) tile[rr * kColTileX + lane] = src_row(oy - ky + rr);
+  for (int rr = wave; rr < nrows; rr += kColWaves) {
+    const int row = oy - ky + rr;
+    const float v = src_raw(row);
+    tile[rr * kColTileX + lane] = src_ok(row) ? v : 0.0f;
+  }
   __syncthreads();
 
+  Epi<EPI> acc;
+  unsigned Wb = (unsigned)W * 4u;
   for (int ty0 = 0; ty0 < H; ty0 += kColTileY) {
+    asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
     const bool more = ty0 + kColTileY < H;
-    if (more) {
-      const int s1 = ty0 + kColTileY + oy - ky + Lp;
-#pragma unroll
-      for (int k = 0; k < kNew; ++k) pre[k] = src_row(s1 + wave + kColWaves * k);
-    }
-
-    // the epilogue's operands (unary, stored belief) are loaded now so that
-    // their latency hides behind the filter
     const int y0 = ty0 + wave * kColRows;
+    const unsigned eb0 = (unsigned)(y0 * W + xo) * 4u;
+    // the epilogue's operands (unary, stored belief), loaded now so that their
+    // latency hides behind the filter
     float pu[kColRows], pb[kColRows];
-    if (need_u) {
+    if (kNeedU || kNeedB) {
+      if (ty0 + kColTileY <= H) {
 #pragma unroll
-      for (int i = 0; i < kColRows; ++i) {
-        const bool in = xin && y0 + i < H;
-        pu[i] = dun[in ? r * HW + (long long)(y0 + i) * W + x : 0];
+        for (int i = 0; i < kColRows; ++i) {
+          const unsigned eb = eb0 + (unsigned)i * Wb;
+          if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
+          if (kNeedB) pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kColRows; ++i) {
+          const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
+          if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
+          if (kNeedB) pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
+        }
       }
     }
-    if (need_b) {
+    // the next tile's new rows (in the last tile: clamped rows, unused)
+    const int s1 = ty0 + kColTileY + oy - ky + Lp;
+    float pre[kNew];
 #pragma unroll
-      for (int i = 0; i < kColRows; ++i) {
-        const bool in = xin && y0 + i < H;
-        pb[i] = dst[in ? r * HW + (long long)(y0 + i) * W + x : 0];
-      }
-    }
+    for (int k = 0; k < kNew; ++k) pre[k] = src_raw(s1 + wave + kColWaves * k);
 
     const float *tw = tile + wave * kColRows * kColTileX + lane;
-    float acc[kColRows], buf[kColRows];
+    float a[kColRows], buf[kColRows];
 #pragma unroll
     for (int i = 0; i < kColRows; ++i) {
-      acc[i] = 0.0f;
+      a[i] = 0.0f;
       buf[i] = tw[i * kColTileX];
     }
     for (int t0 = 0; t0 < Lp; t0 += 2 * 8) {
-      col_chunk8<0>(acc, buf, fy, tw, t0);
+      col_chunk8<0>(a, buf, fy, tw, t0);
       if (t0 + 8 >= Lp) break;
-      col_chunk8<8>(acc, buf, fy, tw, t0 + 8);
+      col_chunk8<8>(a, buf, fy, tw, t0 + 8);
     }
 
 #pragma unroll
     for (int i = 0; i < kColRows; ++i) {
       const int y = y0 + i;
-      if (!xin || y >= H) continue;
-      const int ysrc = y + oy;
       // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
-      const float msg = (col_ok && ysrc >= 0 && ysrc < H) ? log0(acc[i]) + M : kLogZero;
-      const long long e = r * HW + (long long)y * W + x;
-      dst_store(J, dst, dst2, e, (unsigned)e, xmask, msg, need_u ? pu[i] : 0.0f, need_b ? pb[i] : 0.0f, da);
+      const float msg = (col_ok && (unsigned)(y + oy) < (unsigned)H) ? log0(a[i]) + M : kLogZero;
+      if (xin && y < H)
+        acc.store(dst, dst2, eb0 + (unsigned)i * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask,
+                  leaf_u);
     }
     if (more) {
       // move rows [64, nrows) to [0, Lp), then store the prefetched rows
@@ -449,11 +449,37 @@ __global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restri
         }
       }
 #pragma unroll
-      for (int k = 0; k < kNew; ++k) tile[(Lp + wave + kColWaves * k) * kColTileX + lane] = pre[k];
+      for (int k = 0; k < kNew; ++k) {
+        const int row = s1 + wave + kColWaves * k;
+        tile[(Lp + wave + kColWaves * k) * kColTileX + lane] = src_ok(row) ? pre[k] : 0.0f;
+      }
       __syncthreads();
     }
   }
-  dst_flush(b, J, dst2 != nullptr, da, lane);
+  acc.flush(b, J, (long long)r * HW, lane);
+}
+
+__global__ __launch_bounds__(256) void k_colpass(Bases b, const MsgJob *__restrict__ jobs,
+                                                 const float *__restrict__ taps, const int *__restrict__ shifts,
+                                                 int R, int xtiles) {
+  extern __shared__ float4 lds4[];
+  float *tile = reinterpret_cast<float *>(lds4);
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
+  const int r = (int)bx / xtiles;
+  const int xt = (int)bx - r * xtiles;
+  if (xt * kColTileX >= J.W) return;
+  if (J.dst_kind == kDstUpRoot)
+    colpass_strip<kEpiUpRoot>(b, J, taps, shifts, R, r, xt, tile);
+  else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
+    colpass_strip<kEpiUpNode>(b, J, taps, shifts, R, r, xt, tile);
+  else if (J.dst_kind == kDstUpNode)
+    colpass_strip<kEpiUpNodeU>(b, J, taps, shifts, R, r, xt, tile);
+  else if (J.dst_base != 0)
+    colpass_strip<kEpiDownLeaf>(b, J, taps, shifts, R, r, xt, tile);
+  else
+    colpass_strip<kEpiDownMid>(b, J, taps, shifts, R, r, xt, tile);
 }
 
 // ----------------------------------------------------------------- k_root

@@ -337,21 +337,12 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
 // operands, then the next tile's ring rows (always 16 loads, clamped into the
 // grid), and the epilogue waits for its operands only while the ring loads
 // stay in flight through it.
-enum : int {
-  kEpiUpRoot = 0,  // message slot of the root: msg
-  kEpiUpNode = 1,  // chain-node belief: msg
-  kEpiUpNodeU = 2, // chain-node belief: msg + masked unary
-  kEpiDownLeaf = 3, // marginal of a leaf: (masked unary or 0) + msg
-  kEpiDownMid = 4,  // marginal: stored belief + msg; next downward source: masked unary + msg
-};
-
 template <int TERMS, int EPI, bool T16>
 __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
                                                    const int *__restrict__ shifts, int R, int r, int xt,
                                                    uint4 *lds) {
-  constexpr bool kNeedU = EPI >= kEpiUpNodeU;  // unary of the part written
-  constexpr bool kNeedB = EPI == kEpiDownMid;  // stored upward belief
-  constexpr bool kDown = EPI >= kEpiDownLeaf;
+  constexpr bool kNeedU = Epi<EPI>::kNeedU;
+  constexpr bool kNeedB = Epi<EPI>::kNeedB;
   const int H = J.H, W = J.W;
   const int KK = J.kk_y;
   const int ky = J.ky;
@@ -482,10 +473,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   const int xo = xin ? x : W - 1;
   unsigned Wb = (unsigned)W * 4u;
 
-  // reductions: max of the written values (chain-node belief, next downward
-  // source); argmax key of the marginal
-  float vmax = -INFINITY;
-  unsigned best_key = 0u, best_eb = 0u;
+  Epi<EPI> acc;
   int base = 0;
   for (int ty0 = 0; ty0 < H; ty0 += 64) {
     // the per-row offsets are recomputed every tile rather than hoisted into
@@ -544,31 +532,8 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
       const float lg = __builtin_amdgcn_logf(c[i]);  // -inf for 0
       const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
       const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
-      if (xin && y < H) {
-        const unsigned eb = eb0 + (unsigned)dy * Wb;
-        if (EPI == kEpiUpRoot || EPI == kEpiUpNode) {
-          at(dst, eb) = msg;
-          if (EPI == kEpiUpNode) vmax = fmaxf(vmax, msg);
-        } else if (EPI == kEpiUpNodeU) {
-          const float vv = msg + (xmask ? kLogZero : pu[i]);
-          at(dst, eb) = vv;
-          vmax = fmaxf(vmax, vv);
-        } else {
-          const float u = xmask ? kLogZero : pu[i];
-          const float mg = (kNeedB ? pb[i] : (leaf_u ? u : 0.0f)) + msg;
-          at(dst, eb) = mg;
-          const unsigned k = f2key(mg);
-          if (k > best_key) {  // rows ascend: the first maximum is kept
-            best_key = k;
-            best_eb = eb;
-          }
-          if (kNeedB) {
-            const float v2 = u + msg;
-            at(dst2, eb) = v2;
-            vmax = fmaxf(vmax, v2);
-          }
-        }
-      }
+      if (xin && y < H)
+        acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask, leaf_u);
     }
     if (more) {
       __syncthreads();  // every wave is done with this tile's ring rows
@@ -580,15 +545,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
       __syncthreads();
     }
   }
-  if (EPI == kEpiUpNode || EPI == kEpiUpNodeU || EPI == kEpiDownMid) {
-    const unsigned k = wave_max_u32(vmax > -INFINITY ? f2key(vmax) : 0u);
-    if (lane == 0 && k) atomicMax(b.max_keys + (kDown ? J.dst2_key : J.dst_key), k);
-  }
-  if (kDown) {
-    const unsigned long long m =
-        wave_max_u64(best_key ? ((unsigned long long)best_key << 32) | (0xffffffffull - (r * HW + (best_eb >> 2))) : 0ull);
-    if (lane == 0 && m) atomicMax(b.map_keys + J.map_slot, m);
-  }
+  acc.flush(b, J, (long long)r * HW, lane);
 }
 
 template <int TERMS, bool T16>

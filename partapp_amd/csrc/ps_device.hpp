@@ -71,9 +71,6 @@ __device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) 
 
 // log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166)
 __device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
-// the same with the hardware logarithm (v_log_f32 times ln 2), for the BF16
-// passes whose inputs carry 8-bit rounding already
-__device__ __forceinline__ float log0_fast(float v) { return v == 0.0f ? kLogZero : __logf(v); }
 
 // The caller's accumulation of a column pass specialised per destination
 // kind (objectdetect_findrot.cpp:637-654 upward, :201 and :220-222 downward),

@@ -29,7 +29,6 @@
 
 #include "ps_device.hpp"
 #include "ps_internal.hpp"
-#include "ps_rowgather.hpp"
 
 namespace ps {
 

@@ -34,6 +34,7 @@ hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms);
 size_t colpass_mfma_lds_bytes(int KK, int terms);
+size_t colpass_f16_lds_bytes(int KK);
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes);
 hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                                const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
@@ -62,6 +63,7 @@ struct Level {
   bool mfma_ok = false;                // every diagonal job fits the BF16 Toeplitz passes
   size_t mfma_row_lds[3] = {0, 0, 0};  // 1, 2 and 3 terms
   size_t mfma_col_lds[3] = {0, 0, 0};
+  size_t mfma_col16_lds = 0;           // column pass on an fp16 row-pass output
 };
 
 // Host bf16 of a float, round to nearest even (finite inputs).
@@ -221,6 +223,26 @@ struct Builder {
     return off;
   }
 
+  // The same tables in fp16 (hi, then lo = f - hi), for the column pass that
+  // runs on the fp16 MFMA.
+  int add_frags16(const std::vector<float> &f, int KK) {
+    const int off = (int)(frags.size() / 8);
+    const int L = (int)f.size();
+    std::vector<uint16_t> lo;
+    for (int kk = 0; kk < KK; ++kk)
+      for (int l = 0; l < 64; ++l)
+        for (int t = 0; t < 8; ++t) {
+          const int i = 16 * kk + 8 * (l / 32) + t - l % 32;
+          const float v = (i >= 0 && i < L) ? f[i] : 0.0f;
+          const _Float16 hi = (_Float16)v;
+          const _Float16 lv = (_Float16)(v - (float)hi);
+          frags.push_back(__builtin_bit_cast(uint16_t, hi));
+          lo.push_back(__builtin_bit_cast(uint16_t, lv));
+        }
+    frags.insert(frags.end(), lo.begin(), lo.end());
+    return off;
+  }
+
   // Rotation filter of a message as the A operand of the row pass's second
   // GEMM, whose B operand is the x-filter accumulator itself: B position
   // p = 16 kk + 8 h + t (lane half h, element t) holds slice
@@ -297,6 +319,7 @@ struct Builder {
       if (J.kk_x <= kMfmaMaxKK && J.kk_y <= kMfmaMaxKK) {
         J.frag_x = add_frags(mp.fx, J.kk_x);
         J.frag_y = add_frags(mp.fy, J.kk_y);
+        J.frag_y16 = add_frags16(mp.fy, J.kk_y);
         J.frag_r = R <= 24 ? add_rot_frags(mp, R) : 0;
         // the row-pass output is at most (sum of x taps) (largest row sum of
         // the rotation filter) times max exp(b - M) = 1; its fp16 copy holds
@@ -601,10 +624,12 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           for (int t = 0; t < 3 && L.mfma_ok; ++t) {
             L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kx, j.kk_x, t + 1));
             L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t + 1));
+          L.mfma_col16_lds = std::max(L.mfma_col16_lds, colpass_f16_lds_bytes(j.kk_y));
           }
         }
       }
-      if (L.mfma_row_lds[2] > kLdsLimit || L.mfma_col_lds[2] > kLdsLimit) L.mfma_ok = false;
+      if (L.mfma_row_lds[2] > kLdsLimit || L.mfma_col_lds[2] > kLdsLimit || L.mfma_col16_lds > kLdsLimit)
+        L.mfma_ok = false;
       if (L.lds > kLdsLimit || L.col_lds > kLdsLimit)
         return set_error(PS_ERR_UNSUPPORTED, "row pass needs " + std::to_string(L.lds) +
                                                  " bytes of LDS (R * width too large for one workgroup)");
@@ -683,7 +708,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     for (auto &L : *lv)
       if (L.mfma_ok) {
         mrow = std::max(mrow, L.mfma_row_lds[2]);
-        mcol = std::max(mcol, L.mfma_col_lds[2]);
+        mcol = std::max(mcol, std::max(L.mfma_col_lds[2], L.mfma_col16_lds));
       }
   if (e == hipSuccess) e = mfma_set_lds_limit(mrow, mcol);
   if (e != hipSuccess) {
@@ -817,7 +842,7 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     ev2 = profile_begin(pp, 3, s);
     if (mf)
       e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->col_terms, pl->terms != 3, L.max_w,
-                              L.mfma_col_lds[tc], s);
+                              pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
     else
       e = launch_colpass(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
     if (e == hipSuccess)

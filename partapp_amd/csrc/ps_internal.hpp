@@ -82,7 +82,7 @@ struct MsgJob {
   int tpitch;  // row pitch (floats) of the row-pass output grid, a multiple of 32
   int frag_r;  // rotation filter as the A operand of the row pass's second GEMM
   int t16_shift;  // BF16 passes with an fp16 row-pass output: stored as value * 2^t16_shift
-  int pad3_;
+  int frag_y16;   // the y taps as fp16 hi / lo fragments (column pass on an fp16 row-pass output)
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the

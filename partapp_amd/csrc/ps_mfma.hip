@@ -33,6 +33,7 @@
 namespace ps {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 namespace {
@@ -46,6 +47,9 @@ __device__ __forceinline__ bf16x8_t ld_frag(const unsigned short *p) {
 __device__ __forceinline__ bf16x8_t ld_frag(const uint4 *p) { return *reinterpret_cast<const bf16x8_t *>(p); }
 __device__ __forceinline__ f32x16_t mfma(bf16x8_t a, bf16x8_t b, f32x16_t c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16_t mfma16(f16x8_t a, f16x8_t b, f32x16_t c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 // eight floats as bf16 hi parts and lo parts (v - float(hi) is exact)
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8_t &hi, bf16x8_t &lo) {
@@ -336,7 +340,7 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
 // operands, then the next tile's ring rows (always 16 loads, clamped into the
 // grid), and the epilogue waits for its operands only while the ring loads
 // stay in flight through it.
-template <int TERMS, int EPI, bool T16>
+template <int TERMS, int EPI>
 __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
                                                    const int *__restrict__ shifts, int R, int r, int xt,
                                                    uint4 *lds) {
@@ -377,17 +381,11 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   const int scol = xt * 64 + lane;
   const int lcol = scol + ox;
   const bool lcol_ok = scol < W && lcol >= 0 && lcol < W;
-  // (T16: the row pass stored fp16 of value * 2^t16_shift)
-  const long long tcol = (long long)r * H * tp + min(max(lcol, 0), W - 1);
-  const float *Tc = cptr(b, J.tmp) + tcol;
-  const _Float16 *Tc16 = reinterpret_cast<const _Float16 *>(cptr(b, J.tmp)) + tcol;
+  const float *Tc = cptr(b, J.tmp) + (long long)r * H * tp + min(max(lcol, 0), W - 1);
   float scale = 1.0f;
   auto load16 = [&](int row0, float (&v)[16]) {  // row0 is wave-uniform
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int row = min(max(row0 + k, 0), H - 1);
-      v[k] = T16 ? (float)Tc16[row * tp] : Tc[row * tp];
-    }
+    for (int k = 0; k < 16; ++k) v[k] = Tc[min(max(row0 + k, 0), H - 1) * tp];
   };
   auto mask16 = [&](int row0, float (&v)[16]) {
     if (!lcol_ok) {
@@ -444,7 +442,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
     float sc, unsc;
     slice_scale(__uint_as_float(cmax[lane]), sc, unsc);
     scale = sc;
-    if (wave == 0) cexp[lane] = log2f(unsc) - (T16 ? (float)J.t16_shift : 0.0f);  // an integer
+    if (wave == 0) cexp[lane] = log2f(unsc);  // an integer in [-60, 60]
   }
 #pragma unroll
   for (int q = 0; q < kFill; ++q) {
@@ -547,7 +545,180 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   acc.flush(b, J, (long long)r * HW, lane);
 }
 
-template <int TERMS, bool T16>
+// ---------------------------------------- column pass on an fp16 row-pass output
+// The methods whose row pass stores fp16 run the y filter on the fp16 MFMA
+// (v_mfma_f32_32x32x16_f16): the data operand is the stored value itself
+// (exact, no split, no per-column rescale: fp16 values convert to FP32
+// exactly), the taps are split into fp16 hi + lo, two products per k-step.
+// The LDS ring holds one fp16 plane, half the bytes of the bf16 hi / lo pair.
+// Block shape, staging order and epilogue as in colpass_mfma_strip.
+template <int EPI>
+__device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
+                                                  const int *__restrict__ shifts, int R, int r, int xt, uint4 *lds) {
+  constexpr bool kNeedU = Epi<EPI>::kNeedU;
+  constexpr bool kNeedB = Epi<EPI>::kNeedB;
+  const int H = J.H, W = J.W;
+  const int KK = J.kk_y;
+  const int ky = J.ky;
+  int tp = J.tpitch;
+  const int nrows = mfma_ring_rows(KK);
+  const int cs = mfma_ring_pitch(KK);
+  uint4 *fr = lds;  // [2][KK][64] fp16 Toeplitz fragments, hi then lo
+  unsigned short *Rg = reinterpret_cast<unsigned short *>(lds + 2 * KK * 64);  // [64][cs] fp16 ring
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint4 *gf = frags + J.frag_y16;
+  for (int i = threadIdx.x; i < 2 * KK * 64; i += blockDim.x) fr[i] = gf[i];
+  const int ox = shifts[J.shifts + 2 * R + r];
+  const int oy = shifts[J.shifts + 3 * R + r];
+  const float M = key2f(b.max_keys[J.src_key]);
+  const int HW = H * W;
+  float *dst = mptr(b, J.dst) + r * HW;
+  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
+  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * HW : dst;
+  const bool leaf_u = J.dst_base == 1;
+  const UnaryMask dm = J.dst_mask;
+
+  // staging: column xt * 64 + lane + ox, 16 rows at a time (loads clamped into
+  // the grid, rows / columns outside it zeroed before they reach LDS)
+  const int scol = xt * 64 + lane;
+  const int lcol = scol + ox;
+  const bool lcol_ok = scol < W && lcol >= 0 && lcol < W;
+  const unsigned short *Tc =
+      reinterpret_cast<const unsigned short *>(cptr(b, J.tmp)) + (long long)r * H * tp + min(max(lcol, 0), W - 1);
+  auto load16 = [&](int row0, unsigned short (&v)[16]) {  // row0 is wave-uniform
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = Tc[min(max(row0 + k, 0), H - 1) * tp];
+  };
+  auto put16 = [&](int row0, int prow, const unsigned short (&v)[16]) {
+    const bool all = lcol_ok && row0 >= 0 && row0 + 16 <= H;
+    unsigned w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const bool o0 = all || (lcol_ok && row0 + 2 * k >= 0 && row0 + 2 * k < H);
+      const bool o1 = all || (lcol_ok && row0 + 2 * k + 1 >= 0 && row0 + 2 * k + 1 < H);
+      w[k] = (o0 ? (unsigned)v[2 * k] : 0u) | ((o1 ? (unsigned)v[2 * k + 1] : 0u) << 16);
+    }
+    uint4 *dstp = reinterpret_cast<uint4 *>(Rg + lane * cs + prow);
+    dstp[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dstp[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  };
+  // tile rows are source rows ty0 + oy - ky + [0, nrows), at ring row (base +
+  // row) mod nrows
+  constexpr int kFill = 3;  // chunks of 16 rows per thread in the first fill (nrows <= 160)
+  unsigned short fill[kFill][16];
+#pragma unroll
+  for (int q = 0; q < kFill; ++q) {
+    const int c = wave + 4 * q;
+    if (c < nrows / 16) load16(oy - ky + 16 * c, fill[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < kFill; ++q) {
+    const int c = wave + 4 * q;
+    if (c < nrows / 16) put16(oy - ky + 16 * c, 16 * c, fill[q]);
+  }
+  __syncthreads();
+
+  const int cx = (wave & 1) * 32, ry = (wave >> 1) * 32;
+  const int n = lane & 31, h = lane >> 5;
+  const int kq = h << 3;
+  const int x = xt * 64 + cx + n;
+  const bool xin = x < W;
+  const bool col_ok = xin && x + ox >= 0 && x + ox < W;
+  const bool xmask = ((dm.rot_bits >> r) & 1ull) || x < dm.strip || x >= W - dm.strip;
+  const unsigned short *b_col = Rg + (cx + n) * cs;
+  // log(v 2^-s) + M = log2(v) ln 2 + (M - s ln 2), s the row pass's fp16 shift
+  constexpr float kLn2 = 0.693147180559945309f;
+  const float cM = fmaf(-(float)J.t16_shift, kLn2, M);
+  const int xo = xin ? x : W - 1;
+  unsigned Wb = (unsigned)W * 4u;
+
+  Epi<EPI> acc;
+  int base = 0;
+  for (int ty0 = 0; ty0 < H; ty0 += 64) {
+    asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
+    const int yl = ty0 + ry + 4 * h;
+    const unsigned eb0 = (unsigned)(yl * W + xo) * 4u;
+    float pu[16], pb[16];
+    if (kNeedU || kNeedB) {
+      if (ty0 + 64 <= H) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const unsigned eb = eb0 + (unsigned)(8 * (i >> 2) + (i & 3)) * Wb;
+          if (kNeedU) pu[i] = at(dun, eb);
+          if (kNeedB) pb[i] = at(dst, eb);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const unsigned eb = (unsigned)(min(yl + 8 * (i >> 2) + (i & 3), H - 1) * W + xo) * 4u;
+          if (kNeedU) pu[i] = at(dun, eb);
+          if (kNeedB) pb[i] = at(dst, eb);
+        }
+      }
+    }
+    const bool more = ty0 + 64 < H;
+    const int pre_row = ty0 + oy - ky + nrows + 16 * wave;
+    unsigned short pre[16];
+    load16(pre_row, pre);
+
+    f32x16_t c = {};
+    int lrow = base + ry + kq;
+    if (lrow >= nrows) lrow -= nrows;
+    for (int kk = 0; kk < KK; ++kk) {
+      const f16x8_t bv = *reinterpret_cast<const f16x8_t *>(b_col + lrow);
+      c = mfma16(*reinterpret_cast<const f16x8_t *>(fr + kk * 64 + lane), bv, c);
+      c = mfma16(*reinterpret_cast<const f16x8_t *>(fr + (KK + kk) * 64 + lane), bv, c);
+      lrow += 16;
+      if (lrow >= nrows) lrow -= nrows;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int dy = 8 * (i >> 2) + (i & 3);
+      const int y = yl + dy;
+      const float lg = __builtin_amdgcn_logf(c[i]);  // -inf for 0
+      const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
+      const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
+      if (xin && y < H)
+        acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask, leaf_u);
+    }
+    if (more) {
+      __syncthreads();  // every wave is done with this tile's ring rows
+      int prow = base + 16 * wave;
+      if (prow >= nrows) prow -= nrows;
+      put16(pre_row, prow, pre);
+      base += 64;
+      if (base >= nrows) base -= nrows;
+      __syncthreads();
+    }
+  }
+  acc.flush(b, J, (long long)r * HW, lane);
+}
+
+__global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__restrict__ jobs,
+                                                     const uint4 *__restrict__ frags, const int *__restrict__ shifts,
+                                                     int R, int xtiles) {
+  extern __shared__ uint4 lds[];
+  unsigned bx, by;
+  xcd_block(bx, by);
+  const MsgJob &J = jobs[by];
+  const int r = (int)bx / xtiles;
+  const int xt = (int)bx - r * xtiles;
+  if (xt * 64 >= J.W) return;
+  if (J.dst_kind == kDstUpRoot)
+    colpass_f16_strip<kEpiUpRoot>(b, J, frags, shifts, R, r, xt, lds);
+  else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
+    colpass_f16_strip<kEpiUpNode>(b, J, frags, shifts, R, r, xt, lds);
+  else if (J.dst_kind == kDstUpNode)
+    colpass_f16_strip<kEpiUpNodeU>(b, J, frags, shifts, R, r, xt, lds);
+  else if (J.dst_base != 0)
+    colpass_f16_strip<kEpiDownLeaf>(b, J, frags, shifts, R, r, xt, lds);
+  else
+    colpass_f16_strip<kEpiDownMid>(b, J, frags, shifts, R, r, xt, lds);
+}
+
+template <int TERMS>
 __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
                                                       const uint4 *__restrict__ frags, const int *__restrict__ shifts,
                                                       int R, int xtiles) {
@@ -559,15 +730,15 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
   const int xt = (int)bx - r * xtiles;
   if (xt * 64 >= J.W) return;
   if (J.dst_kind == kDstUpRoot)
-    colpass_mfma_strip<TERMS, kEpiUpRoot, T16>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiUpRoot>(b, J, frags, shifts, R, r, xt, lds);
   else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
-    colpass_mfma_strip<TERMS, kEpiUpNode, T16>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiUpNode>(b, J, frags, shifts, R, r, xt, lds);
   else if (J.dst_kind == kDstUpNode)
-    colpass_mfma_strip<TERMS, kEpiUpNodeU, T16>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiUpNodeU>(b, J, frags, shifts, R, r, xt, lds);
   else if (J.dst_base != 0)
-    colpass_mfma_strip<TERMS, kEpiDownLeaf, T16>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiDownLeaf>(b, J, frags, shifts, R, r, xt, lds);
   else
-    colpass_mfma_strip<TERMS, kEpiDownMid, T16>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_mfma_strip<TERMS, kEpiDownMid>(b, J, frags, shifts, R, r, xt, lds);
 }
 
 // ------------------------------------------------------------ host side
@@ -581,6 +752,8 @@ size_t colpass_mfma_lds_bytes(int KK, int terms) {
   return 16 * (size_t)nf * KK * 64 + 2 * (size_t)nd * 64 * mfma_ring_pitch(KK) + 8 * 64;
 }
 
+size_t colpass_f16_lds_bytes(int KK) { return 16 * (size_t)2 * KK * 64 + 2 * (size_t)64 * mfma_ring_pitch(KK); }
+
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   auto set = [](const void *f, size_t bytes) {
     if (bytes <= 64 * 1024) return hipSuccess;
@@ -589,10 +762,8 @@ hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   hipError_t e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<1, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<2, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3, false>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16), col_bytes);
   return e;
 }
 
@@ -620,14 +791,10 @@ hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
   if (terms < 1 || terms > 3 || (!t16 && terms != 3)) return hipErrorInvalidValue;
   const int xtiles = (max_w + 63) / 64;
   dim3 grid(R * xtiles, n_jobs);
-  if (terms == 3 && t16)
-    hipLaunchKernelGGL((k_colpass_mfma<3, true>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
-  else if (terms == 3)
-    hipLaunchKernelGGL((k_colpass_mfma<3, false>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
-  else if (terms == 2)
-    hipLaunchKernelGGL((k_colpass_mfma<2, true>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
-  else
-    hipLaunchKernelGGL((k_colpass_mfma<1, true>), grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+  if (t16)  // fp16 operands: exact data, split taps, whatever the method's terms
+    hipLaunchKernelGGL(k_colpass_f16, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+  else  // an FP32 row-pass output (3-term row pass): bf16 hi / lo split of data and taps
+    hipLaunchKernelGGL(k_colpass_mfma<3>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
   return hipGetLastError();
 }
 

@@ -401,10 +401,12 @@ def main():
                    imgs, busy)}
         print("cpu baseline: %d images in %.2f s on %d threads" % (imgs, busy, cores), file=sys.stderr, flush=True)
 
-    desc = {"bf16mix": "BF16 banded-Toeplitz GEMMs on MFMA (x pass bf16, y pass 3-term split), FP32 accumulate",
-            "bf16x3": "BF16 banded-Toeplitz GEMMs on MFMA, 3-term split, FP32 accumulate",
-            "bf16x2": "BF16 banded-Toeplitz GEMMs on MFMA, data split hi+lo, FP32 accumulate",
-            "bf16": "BF16 banded-Toeplitz GEMMs on MFMA, FP32 accumulate",
+    desc = {"bf16mix": "x, rotation and y filters as MFMA GEMMs (x bf16, rotation bf16 3-term split, fp16 "
+                       "intermediate, y fp16 with split taps), FP32 accumulate",
+            "bf16x3": "x, rotation and y filters as BF16 MFMA GEMMs, 3-term split, FP32 intermediate, FP32 accumulate",
+            "bf16x2": "x, rotation and y filters as MFMA GEMMs (x data split hi+lo, fp16 intermediate), "
+                      "FP32 accumulate",
+            "bf16": "x, rotation and y filters as MFMA GEMMs (x bf16, fp16 intermediate), FP32 accumulate",
             "fp32": "plain FP32"}[args.method]
     out = {
         "metric": METRIC,

@@ -55,8 +55,17 @@ namespace {
 // One level of messages: jobs [off, off + count), the first n_diag with a
 // diagonal covariance (row pass + column pass), the rest off-diagonal (row
 // pass + eigenbasis stages).
+// Diagonal jobs of a level with the same count of 64-column groups
+// (ceil(W / 64)), contiguous after sorting: each class gets its own row-pass
+// and column-pass launches, sized for its widest job, so that the blocks of
+// narrow jobs in an uneven batch do not carry idle waves.
+struct WidthClass {
+  int start = 0, count = 0, max_h = 0, max_w = 0;
+};
+
 struct Level {
   int off = 0, count = 0, n_diag = 0, max_h = 0, max_w = 0;
+  std::vector<WidthClass> classes;  // over the diagonal jobs [0, n_diag)
   size_t lds = 0;      // row pass
   size_t col_lds = 0;  // column pass
   long long od_cells = 0, od_hw = 0;  // max R*th*tw and H*W of the off-diagonal jobs
@@ -604,7 +613,22 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   auto add_levels = [&](std::vector<std::vector<MsgJob>> &lv, std::vector<Level> &dst) -> int {
     for (auto &v : lv) {
       std::stable_partition(v.begin(), v.end(), [](const MsgJob &j) { return j.od == 0; });
+      // diagonal jobs by width class (the order of a level's jobs is free)
+      auto mid = std::find_if(v.begin(), v.end(), [](const MsgJob &j) { return j.od != 0; });
+      std::stable_sort(v.begin(), mid, [](const MsgJob &a, const MsgJob &b2) { return (a.W + 63) / 64 < (b2.W + 63) / 64; });
       Level L;
+      for (auto it = v.begin(); it != mid; ++it) {
+        const int i = (int)(it - v.begin());
+        if (L.classes.empty() || (L.classes.back().max_w + 63) / 64 != (it->W + 63) / 64) {
+          WidthClass c;
+          c.start = i;
+          L.classes.push_back(c);
+        }
+        WidthClass &c = L.classes.back();
+        ++c.count;
+        c.max_h = std::max(c.max_h, it->H);
+        c.max_w = std::max(c.max_w, it->W);
+      }
       L.off = (int)B.jobs.size();
       L.count = (int)v.size();
       L.mfma_ok = R == 24;
@@ -826,28 +850,32 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     const int ti = pl->terms > 0 ? pl->terms - 1 : 0;
     const int tc = pl->col_terms > 0 ? pl->col_terms - 1 : 0;
     const MsgJob *jobs = pl->jobs + L.off;
+    const int n_od = L.count - L.n_diag;
     long ev2 = profile_begin(pp, 2, s);
-    if (mf) {
-      e = launch_rowpass_mfma(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms, pl->mfma_rows,
-                              L.max_h, L.max_w, L.mfma_row_lds[ti], s);
-      if (e == hipSuccess)
-        e = launch_rowpass(b, jobs + L.n_diag, L.count - L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w,
-                           L.lds, s);
-    } else {
-      e = launch_rowpass(b, jobs, L.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
+    for (const WidthClass &c : L.classes) {
+      if (e != hipSuccess) break;
+      if (mf)
+        e = launch_rowpass_mfma(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms,
+                                pl->mfma_rows, c.max_h, c.max_w, L.mfma_row_lds[ti], s);
+      else
+        e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s);
     }
+    if (e == hipSuccess)  // off-diagonal messages: the FP32 row pass, then the eigenbasis stages
+      e = launch_rowpass(b, jobs + L.n_diag, n_od, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
     profile_end(pp, ev2, s);
     if (e != hipSuccess) return;
     // the column pass class also times the eigenbasis stages
     ev2 = profile_begin(pp, 3, s);
-    if (mf)
-      e = launch_colpass_mfma(b, jobs, L.n_diag, pl->frags, pl->shifts, pl->R, pl->col_terms, pl->terms != 3, L.max_w,
-                              pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
-    else
-      e = launch_colpass(b, jobs, L.n_diag, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.col_lds, s);
+    for (const WidthClass &c : L.classes) {
+      if (e != hipSuccess) break;
+      if (mf)
+        e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, pl->col_terms,
+                                pl->terms != 3, c.max_w, pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
+      else
+        e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.col_lds, s);
+    }
     if (e == hipSuccess)
-      e = launch_offdiag(b, pl->jobs + L.off + L.n_diag, L.count - L.n_diag, pl->taps, pl->shifts, pl->R, L.od_cells,
-                         L.od_hw, s);
+      e = launch_offdiag(b, jobs + L.n_diag, n_od, pl->taps, pl->shifts, pl->R, L.od_cells, L.od_hw, s);
     profile_end(pp, ev2, s);
   };
   for (const Level &L : pl->up) {

@@ -145,6 +145,32 @@ def test_fp16_intermediate_uneven_widths_and_rotation_copy(gpu, method):
         assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
 
 
+@pytest.mark.parametrize("method", ["bf16mix", "bf16x3"])
+def test_tiny_grids(gpu, method):
+    # grids narrower and shorter than one MFMA tile (down to 1 x 1) in one
+    # batch: every output finite, every MAP inside its grid, and the
+    # non-degenerate 20 x 24 instance within the budget.  On the smaller ones
+    # the joints shift every message out of the grid, so the normalised FP64
+    # marginals are no reference there (the FP32 path differs from FP64 by
+    # ~0.1 as well).
+    ep = ExpParam(-180.0, 180.0, 24, 0.3, 0.3, 1, 0.0)
+    sizes = [(1, 1), (3, 2), (9, 11), (20, 24)]
+    joints = [synth_joints(40 + i) for i in range(len(sizes))]
+    model = person_model(ep, joints[0])
+    us = [synth_unary(50 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(sizes)]
+    outs, (levels, used) = run_method(model, us, joints, method)
+    assert used == levels
+    for i, (h, w) in enumerate(sizes):
+        marg, root, maps = outs[i]
+        assert np.isfinite(marg).all() and np.isfinite(root).all(), sizes[i]
+        for p in range(10):
+            m = maps[0, p]
+            assert 0 <= int(m["rotidx"]) < 24 and 0 <= int(m["y"]) < h and 0 <= int(m["x"]) < w
+    worst, map_bad = errors_vs_f64(model, us[3], joints[3], outs[3])
+    say("%s tiny grids: 20x24 worst rel L1 vs f64 %.3g, MAP mismatches %s" % (method, worst, map_bad))
+    assert worst <= BUDGET_L1 and not map_bad
+
+
 def test_bf16_graph_equals_eager(gpu):
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
     joints = synth_joints(3)

@@ -31,7 +31,7 @@ hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
                           int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
-hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
+hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, hipStream_t s);
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms);
 size_t colpass_mfma_lds_bytes(int KK, int terms);
 size_t colpass_f16_lds_bytes(int KK);
@@ -144,6 +144,7 @@ struct ps_plan {
   int n_map = 0;
   std::vector<Level> up, down;
   int n_roots = 0, n_maxes = 0;
+  int root_children = 0;  // the same for every root job
   int max_rows = 0, max_hw = 0;
   // graph cache
   hipGraphExec_t gexec = nullptr;
@@ -669,6 +670,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     return st;
   }
   pl->n_roots = (int)B.roots.size();
+  pl->root_children = B.roots.empty() ? 0 : B.roots[0].K;
   pl->n_maxes = (int)B.maxes.size();
   pl->n_keys = items * 2 * P;
   pl->n_map = items * P;
@@ -884,7 +886,7 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
   }
   if (e == hipSuccess) {
     ev = profile_begin(pp, 4, s);
-    e = launch_root(b, pl->roots, pl->n_roots, pl->R, pl->max_hw, s);
+    e = launch_root(b, pl->roots, pl->n_roots, pl->root_children, pl->R, pl->max_hw, s);
     profile_end(pp, ev, s);
   }
   for (const Level &L : pl->down) {

@@ -117,10 +117,12 @@ __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ri
 // grid) and every slice's masking is branch-free.  Waves whose tiles lie past
 // a job's width (the block is sized for the widest job of the launch) only
 // gather and take part in the barriers.
+//
+// At most 72 VGPRs (7 waves per SIMD): five 320-thread blocks per CU for a
+// 300-wide grid instead of four, at the cost of two spilled VGPRs.
 template <int R, int TERMS, bool T16>
-__global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__restrict__ jobs,
-                                                      const int *__restrict__ shifts, const uint4 *__restrict__ frags,
-                                                      int rows) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k_rowpass_mfma(
+    Bases b, const MsgJob *__restrict__ jobs, const int *__restrict__ shifts, const uint4 *__restrict__ frags, int rows) {
   static_assert(R <= 24, "the rotation GEMM keeps output slices 0..23");
   extern __shared__ uint4 lds[];
   unsigned bx, by;
@@ -132,7 +134,6 @@ __global__ __launch_bounds__(512) void k_rowpass_mfma(Bases b, const MsgJob *__r
   const int y1 = min(H, y0 + rows);
   const int kx = J.kx, KK = J.kk_x;
   constexpr int nf = TERMS == 3 ? 2 : 1;  // x tap parts
-  constexpr int nd = TERMS >= 2 ? 2 : 1;  // data parts
   constexpr int nr = 2;                   // rotation tap parts (always hi + lo)
   const int pitch = mfma_row_pitch(W, kx, KK);
   uint4 *fr = lds;                 // [nf][KK][64] Toeplitz fragments

@@ -110,6 +110,7 @@ SIGNATURES = {
     "ps_plan_method_levels": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
     "ps_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "ps_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_plan_set_profiling_classes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "ps_plan_read_profile": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_int_p]),
     "ps_marginals": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ps_instance), ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ps_map), ctypes.c_void_p]),

@@ -217,7 +217,7 @@ class Plan:
     def set_method(self, method):
         """Arithmetic of the spatial passes: 'fp32' (the drop-in), 'bf16',
         'bf16x2', 'bf16x3' or 'bf16mix' (banded-Toeplitz GEMMs on the matrix
-        cores; 'bf16mix' = x pass 'bf16', y pass 'bf16x3').  Returns
+        cores; 'bf16mix' runs the kernels of 'bf16').  Returns
         (levels, levels that use the method)."""
         check(lib().ps_plan_set_method(self.handle, self.METHODS[method]), "ps_plan_set_method")
         a, b = ctypes.c_int(), ctypes.c_int()
@@ -251,8 +251,17 @@ class Plan:
         self.tune(self.TUNE_MFMA_ROWS, best)
         return times
 
-    def set_profiling(self, enable=True):
-        check(lib().ps_plan_set_profiling(self.handle, int(bool(enable))), "ps_plan_set_profiling")
+    def set_profiling(self, enable=True, classes=None):
+        """Bracket the launches with HIP events: every kernel class, or only
+        those named in `classes` (a subset of KERNEL_CLASSES)."""
+        if classes is None:
+            check(lib().ps_plan_set_profiling(self.handle, int(bool(enable))), "ps_plan_set_profiling")
+            return
+        mask = 0
+        for c in classes:
+            mask |= 1 << self.KERNEL_CLASSES.index(c)
+        check(lib().ps_plan_set_profiling_classes(self.handle, ctypes.c_uint(mask if enable else 0)),
+              "ps_plan_set_profiling_classes")
 
     def read_profile(self):
         """{kernel class: (summed ms, launches)} of the profiled executions since the last read."""

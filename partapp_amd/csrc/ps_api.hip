@@ -40,7 +40,7 @@ hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
                                const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
                                size_t lds_bytes, hipStream_t s);
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
-                               int R, int terms, bool t16, int max_w, size_t lds_bytes, hipStream_t s);
+                               int R, int terms, bool t16, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
 }  // namespace ps
 
@@ -61,6 +61,7 @@ namespace {
 // narrow jobs in an uneven batch do not carry idle waves.
 struct WidthClass {
   int start = 0, count = 0, max_h = 0, max_w = 0;
+  int epi_hi = kEpiUpRoot;  // highest epilogue kind of its jobs
 };
 
 struct Level {
@@ -150,8 +151,8 @@ struct ps_plan {
   hipGraphExec_t gexec = nullptr;
   const void *gptr[4] = {nullptr, nullptr, nullptr, nullptr};
   hipStream_t gstream = nullptr;
-  // profiling: event pairs per launch, pooled
-  bool profiling = false;
+  // profiling: event pairs per launch of the classes in the mask, pooled
+  unsigned profile_mask = 0u;
   std::vector<hipEvent_t> events;
   std::vector<int> event_class;  // class of each recorded pair
   size_t n_recorded = 0;         // pairs recorded since the last read
@@ -162,7 +163,7 @@ constexpr int kProfileClasses = 6;
 
 // Record the start event of a launch of class `cls`; returns the pair index or -1.
 long profile_begin(ps_plan *pl, int cls, hipStream_t s) {
-  if (!pl || !pl->profiling) return -1;
+  if (!pl || !((pl->profile_mask >> cls) & 1u)) return -1;
   size_t i = pl->n_recorded;
   while (pl->events.size() < 2 * (i + 1)) {
     hipEvent_t e;
@@ -233,22 +234,22 @@ struct Builder {
     return off;
   }
 
-  // The same tables in fp16 (hi, then lo = f - hi), for the column pass that
-  // runs on the fp16 MFMA.
-  int add_frags16(const std::vector<float> &f, int KK) {
+  // The y taps in fp16 (hi, then lo = f - hi) for the column pass that runs
+  // on the fp16 MFMA, as the 8 shifted copies of f16_tap_copy_len: part p,
+  // copy j, element e holds tap j + e - 32 (zero outside the filter).
+  int add_taps16(const std::vector<float> &f, int KK) {
     const int off = (int)(frags.size() / 8);
-    const int L = (int)f.size();
+    const int L = (int)f.size(), Q = f16_tap_copy_len(KK);
     std::vector<uint16_t> lo;
-    for (int kk = 0; kk < KK; ++kk)
-      for (int l = 0; l < 64; ++l)
-        for (int t = 0; t < 8; ++t) {
-          const int i = 16 * kk + 8 * (l / 32) + t - l % 32;
-          const float v = (i >= 0 && i < L) ? f[i] : 0.0f;
-          const _Float16 hi = (_Float16)v;
-          const _Float16 lv = (_Float16)(v - (float)hi);
-          frags.push_back(__builtin_bit_cast(uint16_t, hi));
-          lo.push_back(__builtin_bit_cast(uint16_t, lv));
-        }
+    for (int j = 0; j < 8; ++j)
+      for (int e = 0; e < Q; ++e) {
+        const int i = j + e - 32;
+        const float v = (i >= 0 && i < L) ? f[i] : 0.0f;
+        const _Float16 hi = (_Float16)v;
+        const _Float16 lv = (_Float16)(v - (float)hi);
+        frags.push_back(__builtin_bit_cast(uint16_t, hi));
+        lo.push_back(__builtin_bit_cast(uint16_t, lv));
+      }
     frags.insert(frags.end(), lo.begin(), lo.end());
     return off;
   }
@@ -329,7 +330,7 @@ struct Builder {
       if (J.kk_x <= kMfmaMaxKK && J.kk_y <= kMfmaMaxKK) {
         J.frag_x = add_frags(mp.fx, J.kk_x);
         J.frag_y = add_frags(mp.fy, J.kk_y);
-        J.frag_y16 = add_frags16(mp.fy, J.kk_y);
+        J.frag_y16 = add_taps16(mp.fy, J.kk_y);
         J.frag_r = R <= 24 ? add_rot_frags(mp, R) : 0;
         // the row-pass output is at most (sum of x taps) (largest row sum of
         // the rotation filter) times max exp(b - M) = 1; its fp16 copy holds
@@ -629,6 +630,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         ++c.count;
         c.max_h = std::max(c.max_h, it->H);
         c.max_w = std::max(c.max_w, it->W);
+        c.epi_hi = std::max(c.epi_hi, epi_kind(*it));
       }
       L.off = (int)B.jobs.size();
       L.count = (int)v.size();
@@ -646,10 +648,12 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           // (the BF16 passes address a row-pass output grid with 32-bit byte offsets)
           if (j.kk_x < 1 || j.kk_y < 1 || j.W > 512 || j.H > 32767 || 4ll * R * j.H * j.tpitch >= (1ll << 32))
             L.mfma_ok = false;
-          for (int t = 0; t < 3 && L.mfma_ok; ++t) {
-            L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kx, j.kk_x, t + 1));
-            L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t + 1));
-          L.mfma_col16_lds = std::max(L.mfma_col16_lds, colpass_f16_lds_bytes(j.kk_y));
+          if (L.mfma_ok) {
+            for (int t = 0; t < 3; ++t) {
+              L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kx, j.kk_x, t + 1));
+              L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t + 1));
+            }
+            L.mfma_col16_lds = std::max(L.mfma_col16_lds, colpass_f16_lds_bytes(j.kk_y));
           }
         }
       }
@@ -792,8 +796,13 @@ extern "C" int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma
 }
 
 extern "C" int ps_plan_set_profiling(ps_plan *plan, int enable) {
+  return ps_plan_set_profiling_classes(plan, enable ? (1u << kProfileClasses) - 1u : 0u);
+}
+
+extern "C" int ps_plan_set_profiling_classes(ps_plan *plan, unsigned mask) {
   if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
-  plan->profiling = enable != 0;
+  if (mask >> kProfileClasses) return set_error(PS_ERR_INVALID_ARG, "profile mask has bits above the kernel classes");
+  plan->profile_mask = mask;
   return PS_OK;
 }
 
@@ -872,7 +881,8 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
       if (e != hipSuccess) break;
       if (mf)
         e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, pl->col_terms,
-                                pl->terms != 3, c.max_w, pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
+                                pl->terms != 3, c.epi_hi, c.max_w,
+                                pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
       else
         e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.col_lds, s);
     }
@@ -916,7 +926,7 @@ extern "C" int ps_plan_execute(ps_plan *plan, const float *log_unary, float *log
   int st = check_exec_args(plan, log_unary, log_marginal);
   if (st != PS_OK) return st;
   return enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, static_cast<hipStream_t>(stream),
-                 plan->profiling);
+                 plan->profile_mask != 0u);
 }
 
 extern "C" int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, float *log_marginal,

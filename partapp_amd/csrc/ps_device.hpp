@@ -73,15 +73,8 @@ __device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) 
 __device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
 
 // The caller's accumulation of a column pass specialised per destination
-// kind (objectdetect_findrot.cpp:637-654 upward, :201 and :220-222 downward),
-// so that a kernel's loads for it are unconditional straight-line code.
-enum : int {
-  kEpiUpRoot = 0,    // message slot of the root: msg
-  kEpiUpNode = 1,    // chain-node belief: msg
-  kEpiUpNodeU = 2,   // chain-node belief: msg + masked unary
-  kEpiDownLeaf = 3,  // marginal of a leaf: (masked unary or 0) + msg
-  kEpiDownMid = 4,   // marginal: stored belief + msg; next downward source: masked unary + msg
-};
+// kind (EpiKind), so that a kernel's loads for it are unconditional
+// straight-line code.
 template <int EPI>
 struct Epi {
   static constexpr bool kNeedU = EPI >= kEpiUpNodeU;  // reads the unary of the part written

@@ -82,7 +82,7 @@ struct MsgJob {
   int tpitch;  // row pitch (floats) of the row-pass output grid, a multiple of 32
   int frag_r;  // rotation filter as the A operand of the row pass's second GEMM
   int t16_shift;  // BF16 passes with an fp16 row-pass output: stored as value * 2^t16_shift
-  int frag_y16;   // the y taps as fp16 hi / lo fragments (column pass on an fp16 row-pass output)
+  int frag_y16;   // the y taps as fp16 hi / lo tap copies (column pass on an fp16 row-pass output)
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the
@@ -98,6 +98,32 @@ struct MsgJob {
   double t_fwd[6];   // rows 0-1; direct: grid -> eigenbasis, bilinear: eigenbasis -> grid
   double t_back[6];  // rows 0-1; grid pixel -> eigenbasis sample position
 };
+
+// The caller's accumulation of a message's column pass
+// (objectdetect_findrot.cpp:637-654 upward, :201 and :220-222 downward), in
+// increasing order of the grids it touches
+enum EpiKind : int {
+  kEpiUpRoot = 0,    // message slot of the root: msg
+  kEpiUpNode = 1,    // chain-node belief: msg
+  kEpiUpNodeU = 2,   // chain-node belief: msg + masked unary
+  kEpiDownLeaf = 3,  // marginal of a leaf: (masked unary or 0) + msg
+  kEpiDownMid = 4,   // marginal: stored belief + msg; next downward source: masked unary + msg
+};
+constexpr int epi_kind(const MsgJob &J) {
+  return J.dst_kind == kDstUpRoot ? kEpiUpRoot
+         : J.dst_kind == kDstUpNode ? (J.dst_add_unary ? kEpiUpNodeU : kEpiUpNode)
+         : J.dst_base != 0 ? kEpiDownLeaf
+                           : kEpiDownMid;
+}
+
+// The y taps of the column pass on the fp16 MFMA (k-steps KK) as 8 copies of
+// the zero-padded taps g[p] = f[p - 32], copy j holding g[j], g[j + 1], ...:
+// the 8 taps of any lane's A fragment are then one aligned 16-byte read.
+// Elements per copy: at least 16 KK + 32, padded to a byte length of 64
+// modulo 256 so that the copies start in different LDS banks.
+constexpr int f16_tap_copy_len(int KK) {
+  return (256 * ((2 * (16 * KK + 32) - 64 + 255) / 256) + 64) / 2;
+}
 
 // floats of the eigenbasis scratch of one message
 inline long long od_scratch_floats(int R, int th, int tw) { return 3ll * R * th * tw; }

@@ -35,6 +35,7 @@ namespace ps {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -564,13 +565,14 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   int tp = J.tpitch;
   const int nrows = mfma_ring_rows(KK);
   const int cs = mfma_ring_pitch(KK);
-  uint4 *fr = lds;  // [2][KK][64] fp16 Toeplitz fragments, hi then lo
-  unsigned short *Rg = reinterpret_cast<unsigned short *>(lds + 2 * KK * 64);  // [64][cs] fp16 ring
+  const int q8 = f16_tap_copy_len(KK) / 8;  // 16-byte units per tap copy
+  uint4 *fr = lds;  // [2][8][q8] fp16 tap copies, hi then lo
+  unsigned short *Rg = reinterpret_cast<unsigned short *>(lds + 16 * q8);  // [64][cs] fp16 ring
 
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint4 *gf = frags + J.frag_y16;
-  for (int i = threadIdx.x; i < 2 * KK * 64; i += blockDim.x) fr[i] = gf[i];
+  for (int i = threadIdx.x; i < 16 * q8; i += blockDim.x) fr[i] = gf[i];
   const int ox = shifts[J.shifts + 2 * R + r];
   const int oy = shifts[J.shifts + 3 * R + r];
   const float M = key2f(b.max_keys[J.src_key]);
@@ -586,29 +588,36 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   const int scol = xt * 64 + lane;
   const int lcol = scol + ox;
   const bool lcol_ok = scol < W && lcol >= 0 && lcol < W;
-  const unsigned short *Tc =
-      reinterpret_cast<const unsigned short *>(cptr(b, J.tmp)) + (long long)r * H * tp + min(max(lcol, 0), W - 1);
-  auto load16 = [&](int row0, unsigned short (&v)[16]) {  // row0 is wave-uniform
+  // rows in pairs as fp16 vectors: moved and masked as values (selects, no
+  // arithmetic), which holds fewer registers than 16 zero-extended halves
+  const _Float16 *Th =
+      reinterpret_cast<const _Float16 *>(cptr(b, J.tmp)) + (long long)r * H * tp + min(max(lcol, 0), W - 1);
+  auto load16 = [&](int row0, h2_t (&v)[8]) {  // row0 is wave-uniform
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = Tc[min(max(row0 + k, 0), H - 1) * tp];
+    for (int k = 0; k < 8; ++k) {
+      h2_t t;
+      t.x = Th[min(max(row0 + 2 * k, 0), H - 1) * tp];
+      t.y = Th[min(max(row0 + 2 * k + 1, 0), H - 1) * tp];
+      v[k] = t;
+    }
   };
-  auto put16 = [&](int row0, int prow, const unsigned short (&v)[16]) {
+  auto put16 = [&](int row0, int prow, const h2_t (&v)[8]) {
     const bool all = lcol_ok && row0 >= 0 && row0 + 16 <= H;
-    unsigned w[8];
+    h2_t *dstp = reinterpret_cast<h2_t *>(Rg + lane * cs + prow);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const bool o0 = all || (lcol_ok && row0 + 2 * k >= 0 && row0 + 2 * k < H);
       const bool o1 = all || (lcol_ok && row0 + 2 * k + 1 >= 0 && row0 + 2 * k + 1 < H);
-      w[k] = (o0 ? (unsigned)v[2 * k] : 0u) | ((o1 ? (unsigned)v[2 * k + 1] : 0u) << 16);
+      h2_t m = v[k];
+      m.x = o0 ? m.x : (_Float16)0.0f;
+      m.y = o1 ? m.y : (_Float16)0.0f;
+      dstp[k] = m;
     }
-    uint4 *dstp = reinterpret_cast<uint4 *>(Rg + lane * cs + prow);
-    dstp[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    dstp[1] = make_uint4(w[4], w[5], w[6], w[7]);
   };
   // tile rows are source rows ty0 + oy - ky + [0, nrows), at ring row (base +
   // row) mod nrows
   constexpr int kFill = 3;  // chunks of 16 rows per thread in the first fill (nrows <= 160)
-  unsigned short fill[kFill][16];
+  h2_t fill[kFill][8];
 #pragma unroll
   for (int q = 0; q < kFill; ++q) {
     const int c = wave + 4 * q;
@@ -629,6 +638,10 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   const bool col_ok = xin && x + ox >= 0 && x + ox < W;
   const bool xmask = ((dm.rot_bits >> r) & 1ull) || x < dm.strip || x >= W - dm.strip;
   const unsigned short *b_col = Rg + (cx + n) * cs;
+  // A fragment of k-step kk: taps 16 kk + c + t - 32 (t = 0..7), c = 8 h + 32 - n,
+  // read from the copy shifted by c mod 8 at the aligned element 16 kk + 8 (c / 8)
+  const int ca = 8 * h + 32 - n;
+  const uint4 *fa = fr + (ca & 7) * q8 + (ca >> 3);
   // log(v 2^-s) + M = log2(v) ln 2 + (M - s ln 2), s the row pass's fp16 shift
   constexpr float kLn2 = 0.693147180559945309f;
   const float cM = fmaf(-(float)J.t16_shift, kLn2, M);
@@ -661,7 +674,7 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
     }
     const bool more = ty0 + 64 < H;
     const int pre_row = ty0 + oy - ky + nrows + 16 * wave;
-    unsigned short pre[16];
+    h2_t pre[8];
     load16(pre_row, pre);
 
     f32x16_t c = {};
@@ -669,8 +682,8 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
     if (lrow >= nrows) lrow -= nrows;
     for (int kk = 0; kk < KK; ++kk) {
       const f16x8_t bv = *reinterpret_cast<const f16x8_t *>(b_col + lrow);
-      c = mfma16(*reinterpret_cast<const f16x8_t *>(fr + kk * 64 + lane), bv, c);
-      c = mfma16(*reinterpret_cast<const f16x8_t *>(fr + (KK + kk) * 64 + lane), bv, c);
+      c = mfma16(*reinterpret_cast<const f16x8_t *>(fa + 2 * kk), bv, c);
+      c = mfma16(*reinterpret_cast<const f16x8_t *>(fa + 8 * q8 + 2 * kk), bv, c);
       lrow += 16;
       if (lrow >= nrows) lrow -= nrows;
     }
@@ -697,6 +710,11 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   acc.flush(b, J, (long long)r * HW, lane);
 }
 
+// HI: the highest epilogue kind among the launch's jobs.  The kernel holds
+// only the kinds up to HI, so a launch without the five-grid downward
+// epilogue is not held to its register count (UpRoot 86, UpNodeU 94,
+// DownLeaf 98, DownMid 116 VGPRs + AGPRs).
+template <int HI>
 __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__restrict__ jobs,
                                                      const uint4 *__restrict__ frags, const int *__restrict__ shifts,
                                                      int R, int xtiles) {
@@ -707,13 +725,14 @@ __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__re
   const int r = (int)bx / xtiles;
   const int xt = (int)bx - r * xtiles;
   if (xt * 64 >= J.W) return;
-  if (J.dst_kind == kDstUpRoot)
+  const int kind = epi_kind(J);
+  if (HI == kEpiUpRoot || kind == kEpiUpRoot)
     colpass_f16_strip<kEpiUpRoot>(b, J, frags, shifts, R, r, xt, lds);
-  else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
+  else if (HI == kEpiUpNode || kind == kEpiUpNode)
     colpass_f16_strip<kEpiUpNode>(b, J, frags, shifts, R, r, xt, lds);
-  else if (J.dst_kind == kDstUpNode)
+  else if (HI == kEpiUpNodeU || kind == kEpiUpNodeU)
     colpass_f16_strip<kEpiUpNodeU>(b, J, frags, shifts, R, r, xt, lds);
-  else if (J.dst_base != 0)
+  else if (HI == kEpiDownLeaf || kind == kEpiDownLeaf)
     colpass_f16_strip<kEpiDownLeaf>(b, J, frags, shifts, R, r, xt, lds);
   else
     colpass_f16_strip<kEpiDownMid>(b, J, frags, shifts, R, r, xt, lds);
@@ -753,7 +772,9 @@ size_t colpass_mfma_lds_bytes(int KK, int terms) {
   return 16 * (size_t)nf * KK * 64 + 2 * (size_t)nd * 64 * mfma_ring_pitch(KK) + 8 * 64;
 }
 
-size_t colpass_f16_lds_bytes(int KK) { return 16 * (size_t)2 * KK * 64 + 2 * (size_t)64 * mfma_ring_pitch(KK); }
+size_t colpass_f16_lds_bytes(int KK) {
+  return 2 * 8 * 2 * (size_t)f16_tap_copy_len(KK) + 2 * (size_t)64 * mfma_ring_pitch(KK);
+}
 
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   auto set = [](const void *f, size_t bytes) {
@@ -764,7 +785,11 @@ hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid>), col_bytes);
   return e;
 }
 
@@ -785,16 +810,25 @@ hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
   return hipGetLastError();
 }
 
-// t16: the row pass stored its output as fp16 (row-pass terms 1 and 2)
+// t16: the row pass stored its output as fp16 (row-pass terms 1 and 2);
+// epi_hi: the highest epilogue kind (EpiKind) among the jobs
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
-                               int R, int terms, bool t16, int max_w, size_t lds_bytes, hipStream_t s) {
+                               int R, int terms, bool t16, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
-  if (terms < 1 || terms > 3 || (!t16 && terms != 3)) return hipErrorInvalidValue;
+  if (terms < 1 || terms > 3 || (!t16 && terms != 3) || epi_hi < kEpiUpRoot || epi_hi > kEpiDownMid)
+    return hipErrorInvalidValue;
   const int xtiles = (max_w + 63) / 64;
   dim3 grid(R * xtiles, n_jobs);
-  if (t16)  // fp16 operands: exact data, split taps, whatever the method's terms
-    hipLaunchKernelGGL(k_colpass_f16, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
-  else  // an FP32 row-pass output (3-term row pass): bf16 hi / lo split of data and taps
+  const dim3 blk(256);
+  if (t16) {  // fp16 operands: exact data, split taps, whatever the method's terms
+    switch (epi_hi) {
+      case kEpiUpRoot: hipLaunchKernelGGL(k_colpass_f16<kEpiUpRoot>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpNode: hipLaunchKernelGGL(k_colpass_f16<kEpiUpNode>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpNodeU: hipLaunchKernelGGL(k_colpass_f16<kEpiUpNodeU>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiDownLeaf: hipLaunchKernelGGL(k_colpass_f16<kEpiDownLeaf>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      default: hipLaunchKernelGGL(k_colpass_f16<kEpiDownMid>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+    }
+  } else  // an FP32 row-pass output (3-term row pass): bf16 hi / lo split of data and taps
     hipLaunchKernelGGL(k_colpass_mfma<3>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
   return hipGetLastError();
 }

@@ -150,6 +150,36 @@ def test_sweep_small_matches_oracle(gpu, seed):
     check_parity("small seed %d" % seed, model, u, joints, out)
 
 
+# trees other than the person tree, by parent of each part (-1 = root): the
+# root's child count selects the root kernel's instantiation (1..8 unrolled,
+# above 8 the count read at run time)
+TREES = {
+    "chain3": [-1, 0, 1],                      # one chain of two below the root
+    "star3": [1, -1, 1, 1],                    # root = part 1, three leaves
+    "mixed4": [-1, 0, 1, 0, 0, 4, 5, 0],       # chains of length 2, 1, 3, 1
+    "star9": [-1, 0, 0, 0, 0, 0, 0, 0, 0, 0],  # nine leaves: the run-time count
+}
+
+
+@pytest.mark.parametrize("tree", sorted(TREES))
+def test_root_child_counts(gpu, tree):
+    parents = TREES[tree]
+    P = len(parents)
+    root = parents.index(-1)
+    base = synth_joints(11) + synth_joints(12)
+    joints = []
+    for c, par in enumerate(parents):
+        if par >= 0:
+            j = base[len(joints)]
+            joints.append(Joint(c, par, j.offset_c, j.offset_p, j.C, j.rot_mean, j.rot_sigma))
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    model = PartModel([1] * P, [0] * P, [1 if p == root else 0 for p in range(P)], joints, ep, True)
+    u = synth_unary(13, P, 1, 24, 112, 128)
+    say("tree", tree, parents)
+    out = run_plan(model, [u])[0]
+    check_parity("tree %s" % tree, model, u, joints, out, check_f64=False)
+
+
 def test_sweep_cfg0_full_size(gpu):
     # one instance, 200 x 200 x 24 rotations, 10 parts, seed 0
     ep = config_exp_param(0)

@@ -209,8 +209,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
     const int x = lane_column();
     float e[R];
     if (zero_src) {
+      // a leaf without a detector keeps the zero log posterior it was
+      // allocated with (no unary added, objectdetect_findrot.cpp:652-654):
+      // exp(0 - M) = 1 (its max M is 0) wherever the warp has a source
+      // pixel, exp(LOG_ZERO) = 0 elsewhere
 #pragma unroll
-      for (int ro = 0; ro < R; ++ro) e[ro] = 0.0f;
+      for (int ro = 0; ro < R; ++ro) {
+        const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)(x + ox[ro]) < (unsigned)W;
+        e[ro] = ok ? 1.0f : 0.0f;
+      }
     } else if (use_mask) {
       // no source pixel, or a masked unary state: LOG_ZERO (objectdetect_findrot.cpp:342-365)
 #pragma unroll

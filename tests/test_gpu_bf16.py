@@ -145,6 +145,35 @@ def test_fp16_intermediate_uneven_widths_and_rotation_copy(gpu, method):
         assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
 
 
+# parts without a detector (is_detect = 0): a leaf whose source is the zero
+# grid (row pass), a chain node's belief without its unary (up), a leaf
+# marginal without one (down).  "mixed" puts them next to the ordinary kinds
+# in one launch; with every chain node undetected the up level's launch holds
+# only the kinds without a unary read.
+DETECT_CASES = {"mixed": (7, 9), "no_mid_unaries": (1, 2, 7, 8)}
+
+
+@pytest.mark.parametrize("method", ["bf16mix", "bf16x3"])
+@pytest.mark.parametrize("case", sorted(DETECT_CASES))
+def test_parts_without_detector(gpu, case, method):
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    is_detect = [1] * 10
+    for p in DETECT_CASES[case]:
+        is_detect[p] = 0
+    joints = [synth_joints(100), synth_joints(101)]
+    model = person_model(ep, joints[0], is_detect=is_detect)
+    sizes = [(104, 120), (96, 136)]
+    us = [synth_unary(110 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(sizes)]
+    say("parts without detector", case, method, is_detect)
+    outs, (levels, used) = run_method(model, us, joints, method)
+    assert used == levels
+    for i in range(len(sizes)):
+        worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
+        say("  %s %s instance %d: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (case, method, i, worst, map_bad))
+        assert worst <= METHOD_L1[method]
+        assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
+
+
 @pytest.mark.parametrize("method", ["bf16mix", "bf16x3"])
 def test_tiny_grids(gpu, method):
     # grids narrower and shorter than one MFMA tile (down to 1 x 1) in one

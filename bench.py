@@ -412,6 +412,7 @@ def main():
         k = min(n, 64)
         imgs, busy, cores = cpu_baseline_timed(ep, sizes[:k], joints[:k], seeds[:k], args.cpu_seconds)
         cpu = {"value": round(imgs / busy, 4), "unit": "images/s", "cores": cores, "kind": "port",
+               "per_core": round(imgs / busy / cores, 4),
                "sample": "%d sweeps over the first instances (float32 oracle, one instance per thread), %.1f s" % (
                    imgs, busy)}
         print("cpu baseline: %d images in %.2f s on %d threads" % (imgs, busy, cores), file=sys.stderr, flush=True)

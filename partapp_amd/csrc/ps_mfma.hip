@@ -1,9 +1,10 @@
-// BF16 banded-Toeplitz variant of a message's spatial filter on the gfx950
-// matrix cores.  Same data flow as the FP32 row / column passes (the forward
-// shift, exp, rotation filter, log, inverse shift and the caller's
-// accumulation are unchanged); only the two truncated Gaussian passes
-// (multi_array_filter.hpp:277-288 along x, :304-317 along y) become GEMMs
-// against a Toeplitz matrix of the taps:
+// Banded-Toeplitz variant of a message's filters on the gfx950 matrix cores.
+// Same data flow as the FP32 row / column passes (the forward shift, exp, log,
+// inverse shift and the caller's accumulation are unchanged); the two
+// truncated Gaussian passes (multi_array_filter.hpp:277-288 along x, :304-317
+// along y) become GEMMs against a Toeplitz matrix of the taps, and the
+// circular rotation filter (objectdetect_findrot.cpp:377-420) a GEMM against
+// its circulant matrix (row pass, below):
 //
 //   out[m][n] = sum_k A[m][k] B[k][n],  band entries f[k - m] (y pass, A) or
 //                                       f[k - n] (x pass, B), zero elsewhere
@@ -12,11 +13,14 @@
 // needs K = 32 + 2k inputs, rounded up to KK steps of 16.  TERMS selects the
 // precision: 1 multiplies bf16(data) by bf16(taps); 2 splits the data into
 // bf16 hi + lo parts (data to about 16 significant bits, taps in bf16); 3
-// splits both and adds hi*hi + lo*hi + hi*lo.  The column pass rescales every
-// column of a slice by a power of two taken from its first rows (the y filter
-// never mixes columns), so the bf16 operands keep their range; the scale is
-// undone exactly after the GEMM.  The exponential of the row pass and the
-// logarithm of the column pass use the hardware instructions (a few ulp).
+// splits both and adds hi*hi + lo*hi + hi*lo.  With TERMS 3 the row pass
+// stores FP32 and the bf16 column pass rescales every column of a slice by a
+// power of two taken from its first rows (the y filter never mixes columns),
+// so the bf16 operands keep their range; the scale is undone exactly after
+// the GEMM.  With TERMS 1 and 2 the row pass stores fp16 scaled by a power of
+// two per message and the column pass runs on the fp16 MFMA.  The exponential
+// of the row pass and the logarithm of the column pass use the hardware
+// instructions (a few ulp).
 //
 // Operand layouts of the 32x32x16 bf16 MFMA (lane l, t = 0..7):
 //   A: A[l % 32][(l / 32) * 8 + t]   B: B[(l / 32) * 8 + t][l % 32]
@@ -559,8 +563,11 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
 // (v_mfma_f32_32x32x16_f16): the data operand is the stored value itself
 // (exact, no split, no per-column rescale: fp16 values convert to FP32
 // exactly), the taps are split into fp16 hi + lo, two products per k-step.
-// The LDS ring holds one fp16 plane, half the bytes of the bf16 hi / lo pair.
-// Block shape, staging order and epilogue as in colpass_mfma_strip.
+// The LDS ring holds one fp16 plane, half the bytes of the bf16 hi / lo pair,
+// and the taps sit in LDS as the 8 shifted copies of f16_tap_copy_len (5 KB
+// at KK = 7 where per-lane fragments take 14 KB), so that LDS no longer
+// limits a CU to 4 blocks.  Block shape, staging order and epilogue as in
+// colpass_mfma_strip.
 template <int EPI>
 __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
                                                   const int *__restrict__ shifts, int R, int r, int xt, uint4 *lds) {

@@ -96,18 +96,20 @@ def workload_flops(msgs, R):
     return row, col
 
 
-def mfma_flops(msgs, R, row_terms, col_terms):
+def mfma_flops(msgs, R, row_terms, col_products):
     """Issued MFMA flops of the Toeplitz GEMMs per sweep (32x32x16 tiles, K
-    rounded up to steps of 16, R = 24 rotation rows padded to 32); the row
-    pass also runs the rotation filter as a 2-step GEMM per tile (data split
-    hi + lo, taps split too for 3 terms)."""
+    rounded up to steps of 16, R = 24 rotation rows padded to 32): the row
+    pass's x filter with row_terms products per k-step, then the rotation
+    filter as a 2-step GEMM per tile with data and taps split (3 products);
+    the column pass with col_products per k-step (fp16 data times split fp16
+    taps: 2; the bf16 three-term pass: 3)."""
     tile = 2.0 * 32 * 32 * 16
     fl = 0.0
     for H, W, rl, kx, ky in msgs:
         kkx, kky = (32 + 2 * kx + 15) // 16, (32 + 2 * ky + 15) // 16
-        rot = 2 * (3 if row_terms == 3 else 2)
+        rot = 2 * 3
         fl += tile * (row_terms * kkx + rot) * ((W + 31) // 32) * H  # row pass: one 32-row tile per output row
-        fl += tile * col_terms * kky * 4 * R * ((W + 63) // 64) * ((H + 63) // 64)  # column pass: 4 waves / 64x64
+        fl += tile * col_products * kky * 4 * R * ((W + 63) // 64) * ((H + 63) // 64)  # column pass: 4 waves / 64x64
     return fl
 
 
@@ -400,7 +402,7 @@ def main():
             if f:
                 roofline["fp32_tflops"] = round(f / (t_dom / args.steps / 1e3) / 1e12, 3)
         else:
-            terms = {"bf16x3": (3, 3), "bf16x2": (2, 2), "bf16": (1, 1), "bf16mix": (1, 3)}[args.method]
+            terms = {"bf16x3": (3, 3), "bf16x2": (2, 2), "bf16": (1, 2), "bf16mix": (1, 2)}[args.method]
             t_gemm = (brk["rowpass"][0] + brk["colpass"][0]) / brk_steps / 1e3
             fl = mfma_flops(msgs, R, *terms)
             roofline["mfma"] = {"issued_tflops": round(fl / t_gemm / 1e12, 2), "peak": BF16_PEAK_TFLOPS,

@@ -56,20 +56,37 @@ __global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__
     } else {
       const float *src = cptr(b, J.src);
       const bool plain = J.mask.rot_bits == 0ull && J.mask.strip == 0;
+      // four independent loads per thread in flight (a lane past the end
+      // reads its first element again, which leaves the max unchanged)
+      constexpr int kU = 4;
       if (plain && ((reinterpret_cast<uintptr_t>(src + e0) & 15) == 0) && ((e1 - e0) & 3) == 0) {
         const float4 *s4 = reinterpret_cast<const float4 *>(src + e0);
         const int n4 = (int)((e1 - e0) >> 2);
-        for (int i = threadIdx.x; i < n4; i += blockDim.x) {
-          const float4 v = s4[i];
-          km = max(km, max(max(f2key(v.x), f2key(v.y)), max(f2key(v.z), f2key(v.w))));
+        for (int i = threadIdx.x; i < n4; i += kU * blockDim.x) {
+          float4 v[kU];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            const int j = i + u * (int)blockDim.x;
+            v[u] = s4[j < n4 ? j : i];
+          }
+#pragma unroll
+          for (int u = 0; u < kU; ++u)
+            km = max(km, max(max(f2key(v[u].x), f2key(v[u].y)), max(f2key(v[u].z), f2key(v[u].w))));
         }
       } else {
         const long long HW = (long long)H * W;
-        for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-          const int r = (int)(e / HW);
-          const int x = (int)(e % W);
-          const float v = masked(J.mask, r, x, W) ? kLogZero : src[e];
-          km = max(km, f2key(v));
+        for (long long e = e0 + threadIdx.x; e < e1; e += kU * blockDim.x) {
+          float v[kU];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            const long long f = e + u * (long long)blockDim.x;
+            const long long g = f < e1 ? f : e;
+            const int r = (int)(g / HW);
+            const int x = (int)(g % W);
+            v[u] = masked(J.mask, r, x, W) ? kLogZero : src[g];
+          }
+#pragma unroll
+          for (int u = 0; u < kU; ++u) km = max(km, f2key(v[u]));
         }
       }
     }

@@ -295,6 +295,44 @@ def test_constant_unaries_ties_take_the_first_index(gpu):
         assert got == tuple(b32[0, p])
 
 
+def test_profiling_classes_bracket_only_their_launches(gpu):
+    # per-kernel timing of every class, or of the column pass alone, leaves
+    # the results unchanged; a mask beyond the six classes is an error
+    import ctypes
+    from partapp_amd._lib import PsError, check, lib
+    torch = torch_mod()
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(4)
+    model = person_model(ep, joints)
+    u = synth_unary(4, 10, 1, 24, 64, 72)
+    plan = Plan(model, [(64, 72)], [joints])
+    dev = torch.device("cuda:0")
+    un = torch.from_numpy(u.ravel()).to(dev)
+    outs = []
+    for classes in (None, Plan.KERNEL_CLASSES, ("colpass",)):
+        marg = torch.full_like(un, float("nan"))
+        root = torch.full((plan.root_numel,), float("nan"), device=dev)
+        maps = torch.zeros((plan.map_numel * 4,), dtype=torch.int32, device=dev)
+        if classes is not None:
+            plan.set_profiling(True, classes=classes)
+            plan.read_profile()
+        plan.execute(un, marg, root, maps)
+        torch.cuda.synchronize()
+        if classes is not None:
+            prof = plan.read_profile()
+            plan.set_profiling(False)
+            for k, (ms, n) in prof.items():
+                assert (n > 0) == (k in classes), (classes, k, n)
+                assert ms >= 0.0
+        outs.append(marg.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    assert prof["colpass"][1] == 4  # two levels up, two down
+    with pytest.raises(PsError):
+        check(lib().ps_plan_set_profiling_classes(plan.handle, ctypes.c_uint(1 << len(Plan.KERNEL_CLASSES))),
+              "ps_plan_set_profiling_classes")
+    plan.close()
+
+
 def test_plan_errors_on_gpu(gpu):
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
     model = person_model(ep)

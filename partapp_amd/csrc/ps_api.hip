@@ -66,7 +66,8 @@ struct WidthClass {
 
 struct Level {
   int off = 0, count = 0, n_diag = 0, max_h = 0, max_w = 0;
-  std::vector<WidthClass> classes;  // over the diagonal jobs [0, n_diag)
+  std::vector<WidthClass> classes;      // row passes: over the diagonal jobs [0, n_diag)
+  std::vector<WidthClass> col_classes;  // column passes: the same split further by epilogue kind
   size_t lds = 0;      // row pass
   size_t col_lds = 0;  // column pass
   long long od_cells = 0, od_hw = 0;  // max R*th*tw and H*W of the off-diagonal jobs
@@ -617,20 +618,31 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       std::stable_partition(v.begin(), v.end(), [](const MsgJob &j) { return j.od == 0; });
       // diagonal jobs by width class (the order of a level's jobs is free)
       auto mid = std::find_if(v.begin(), v.end(), [](const MsgJob &j) { return j.od != 0; });
-      std::stable_sort(v.begin(), mid, [](const MsgJob &a, const MsgJob &b2) { return (a.W + 63) / 64 < (b2.W + 63) / 64; });
+      // and, within a width class, by epilogue kind: the column pass launches
+      // each kind on its own, so a kind that needs fewer registers runs at
+      // its own occupancy and every block of a launch does the same work
+      std::stable_sort(v.begin(), mid, [](const MsgJob &a, const MsgJob &b2) {
+        const int ca = (a.W + 63) / 64, cb = (b2.W + 63) / 64;
+        return ca != cb ? ca < cb : epi_kind(a) < epi_kind(b2);
+      });
       Level L;
-      for (auto it = v.begin(); it != mid; ++it) {
-        const int i = (int)(it - v.begin());
-        if (L.classes.empty() || (L.classes.back().max_w + 63) / 64 != (it->W + 63) / 64) {
+      auto add_to = [](std::vector<WidthClass> &cl, int i, const MsgJob &j, bool split_kind) {
+        if (cl.empty() || (cl.back().max_w + 63) / 64 != (j.W + 63) / 64 ||
+            (split_kind && cl.back().epi_hi != epi_kind(j))) {
           WidthClass c;
           c.start = i;
-          L.classes.push_back(c);
+          cl.push_back(c);
         }
-        WidthClass &c = L.classes.back();
+        WidthClass &c = cl.back();
         ++c.count;
-        c.max_h = std::max(c.max_h, it->H);
-        c.max_w = std::max(c.max_w, it->W);
-        c.epi_hi = std::max(c.epi_hi, epi_kind(*it));
+        c.max_h = std::max(c.max_h, j.H);
+        c.max_w = std::max(c.max_w, j.W);
+        c.epi_hi = std::max(c.epi_hi, epi_kind(j));
+      };
+      for (auto it = v.begin(); it != mid; ++it) {
+        const int i = (int)(it - v.begin());
+        add_to(L.classes, i, *it, false);
+        add_to(L.col_classes, i, *it, true);
       }
       L.off = (int)B.jobs.size();
       L.count = (int)v.size();
@@ -877,7 +889,10 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     if (e != hipSuccess) return;
     // the column pass class also times the eigenbasis stages
     ev2 = profile_begin(pp, 3, s);
-    for (const WidthClass &c : L.classes) {
+    // the fp16 column pass has one instantiation per highest epilogue kind:
+    // it launches per (width, kind) class; the other column passes per width
+    const bool t16 = mf && pl->terms != 3;
+    for (const WidthClass &c : t16 ? L.col_classes : L.classes) {
       if (e != hipSuccess) break;
       if (mf)
         e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, pl->col_terms,

@@ -28,7 +28,7 @@ hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
                           int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
 size_t colpass_lds_bytes(int ky);
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
+                          int R, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s);
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, hipStream_t s);
@@ -874,36 +874,45 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     const int tc = pl->col_terms > 0 ? pl->col_terms - 1 : 0;
     const MsgJob *jobs = pl->jobs + L.off;
     const int n_od = L.count - L.n_diag;
-    long ev2 = profile_begin(pp, 2, s);
+    // every kernel launch (the eigenbasis stages of a level as one) has its
+    // own pair of profiling events
     for (const WidthClass &c : L.classes) {
       if (e != hipSuccess) break;
+      const long ev2 = profile_begin(pp, 2, s);
       if (mf)
         e = launch_rowpass_mfma(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms,
                                 pl->mfma_rows, c.max_h, c.max_w, L.mfma_row_lds[ti], s);
       else
         e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s);
+      profile_end(pp, ev2, s);
     }
-    if (e == hipSuccess)  // off-diagonal messages: the FP32 row pass, then the eigenbasis stages
+    if (e == hipSuccess && n_od > 0) {  // off-diagonal messages: the FP32 row pass, then the eigenbasis stages
+      const long ev2 = profile_begin(pp, 2, s);
       e = launch_rowpass(b, jobs + L.n_diag, n_od, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
-    profile_end(pp, ev2, s);
+      profile_end(pp, ev2, s);
+    }
     if (e != hipSuccess) return;
-    // the column pass class also times the eigenbasis stages
-    ev2 = profile_begin(pp, 3, s);
-    // the fp16 column pass has one instantiation per highest epilogue kind:
-    // it launches per (width, kind) class; the other column passes per width
-    const bool t16 = mf && pl->terms != 3;
-    for (const WidthClass &c : t16 ? L.col_classes : L.classes) {
+    // the fp16 and FP32 column passes have one instantiation per highest
+    // epilogue kind: they launch per (width, kind) class; the bf16 three-term
+    // column pass per width
+    const bool per_kind = !mf || pl->terms != 3;
+    for (const WidthClass &c : per_kind ? L.col_classes : L.classes) {
       if (e != hipSuccess) break;
+      const long ev2 = profile_begin(pp, 3, s);
       if (mf)
         e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, pl->col_terms,
                                 pl->terms != 3, c.epi_hi, c.max_w,
                                 pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
       else
-        e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.col_lds, s);
+        e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s);
+      profile_end(pp, ev2, s);
     }
-    if (e == hipSuccess)
+    // the column pass class also times the eigenbasis stages
+    if (e == hipSuccess && n_od > 0) {
+      const long ev2 = profile_begin(pp, 3, s);
       e = launch_offdiag(b, jobs + L.n_diag, n_od, pl->taps, pl->shifts, pl->R, L.od_cells, L.od_hw, s);
-    profile_end(pp, ev2, s);
+      profile_end(pp, ev2, s);
+    }
   };
   for (const Level &L : pl->up) {
     if (e != hipSuccess) break;
@@ -1090,7 +1099,7 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
     if (mp.od)
       e = launch_offdiag(b, jd, 1, td, sd, R, (long long)R * mp.th * mp.tw, (long long)height * width, s);
     else
-      e = launch_colpass(b, jd, 1, td, sd, R, height, width, colpass_lds_bytes(mp.ky), s);
+      e = launch_colpass(b, jd, 1, td, sd, R, epi_kind(mj), width, colpass_lds_bytes(mp.ky), s);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(mem);

@@ -475,11 +475,15 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   acc.flush(b, J, (long long)r * HW, lane);
 }
 
-// At most 128 VGPRs (4 waves per SIMD instead of 3): a few spilled VGPRs cost
-// less than the lost occupancy (9 % faster on 300x300x24 grids).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_colpass(Bases b, const MsgJob *__restrict__ jobs,
-                                                 const float *__restrict__ taps, const int *__restrict__ shifts,
-                                                 int R, int xtiles) {
+// HI: the highest destination kind among the launch's jobs (the kernel holds
+// only the epilogues up to it: 100-123 VGPRs below the five-grid downward
+// one).  At most 128 VGPRs (4 waves per SIMD instead of 3): the five-grid
+// epilogue spills a few registers, which costs less than the lost occupancy
+// (9 % faster on 300x300x24 grids).
+template <int HI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_colpass(
+    Bases b, const MsgJob *__restrict__ jobs, const float *__restrict__ taps, const int *__restrict__ shifts, int R,
+    int xtiles) {
   extern __shared__ float4 lds4[];
   float *tile = reinterpret_cast<float *>(lds4);
   unsigned bx, by;
@@ -488,13 +492,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int r = (int)bx / xtiles;
   const int xt = (int)bx - r * xtiles;
   if (xt * kColTileX >= J.W) return;
-  if (J.dst_kind == kDstUpRoot)
+  const int kind = epi_kind(J);
+  if (HI == kEpiUpRoot || kind == kEpiUpRoot)
     colpass_strip<kEpiUpRoot>(b, J, taps, shifts, R, r, xt, tile);
-  else if (J.dst_kind == kDstUpNode && !J.dst_add_unary)
+  else if (HI == kEpiUpNode || kind == kEpiUpNode)
     colpass_strip<kEpiUpNode>(b, J, taps, shifts, R, r, xt, tile);
-  else if (J.dst_kind == kDstUpNode)
+  else if (HI == kEpiUpNodeU || kind == kEpiUpNodeU)
     colpass_strip<kEpiUpNodeU>(b, J, taps, shifts, R, r, xt, tile);
-  else if (J.dst_base != 0)
+  else if (HI == kEpiDownLeaf || kind == kEpiDownLeaf)
     colpass_strip<kEpiDownLeaf>(b, J, taps, shifts, R, r, xt, tile);
   else
     colpass_strip<kEpiDownMid>(b, J, taps, shifts, R, r, xt, tile);
@@ -677,13 +682,21 @@ hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 
 size_t colpass_lds_bytes(int ky) { return sizeof(float) * (size_t)col_tile_rows(ky) * kColTileX; }
 
+// epi_hi: the highest destination kind (EpiKind) among the jobs
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s) {
-  (void)max_h;
+                          int R, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
+  if (epi_hi < kEpiUpRoot || epi_hi > kEpiDownMid) return hipErrorInvalidValue;
   const int xtiles = (max_w + kColTileX - 1) / kColTileX;
   dim3 grid(R * xtiles, n_jobs);
-  hipLaunchKernelGGL(k_colpass, grid, dim3(kColWaves * 64), lds_bytes, s, b, jobs, taps, shifts, R, xtiles);
+  const dim3 blk(kColWaves * 64);
+  switch (epi_hi) {
+    case kEpiUpRoot: hipLaunchKernelGGL(k_colpass<kEpiUpRoot>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpNode: hipLaunchKernelGGL(k_colpass<kEpiUpNode>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpNodeU: hipLaunchKernelGGL(k_colpass<kEpiUpNodeU>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiDownLeaf: hipLaunchKernelGGL(k_colpass<kEpiDownLeaf>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    default: hipLaunchKernelGGL(k_colpass<kEpiDownMid>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+  }
   return hipGetLastError();
 }
 

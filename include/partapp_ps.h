@@ -164,7 +164,8 @@ int ps_plan_tune(ps_plan *plan, int knob, int value);
 int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
 
 /* Per-kernel timing of a plan's executions, for measurement: while enabled,
- * every launch of ps_plan_execute is bracketed by HIP events on its stream.
+ * every kernel launch of ps_plan_execute (a level's eigenbasis stages count as
+ * one) is bracketed by HIP events on its stream.
  * ps_plan_read_profile waits for the recorded launches and returns, per kernel
  * class (0 reset, 1 leaf max, 2 row pass, 3 column pass, 4 root combine,
  * 5 MAP decode), the summed milliseconds and the launch count since the last

@@ -6,20 +6,21 @@ per-image spatial model (seeds 0..31).  One step = the whole sweep of the
 batch (upward pass, root, downward pass, marginals of all parts, MAP, root
 posterior) with the unaries already resident in HBM.
 
-The metric compares the chosen method (the spatial Gaussian passes as BF16
-banded-Toeplitz GEMMs on the matrix cores: the x pass bf16 x bf16, the y pass
-with data and taps split into bf16 hi + lo) with the plain FP32 sweep (the
-drop-in) on the same plan and inputs, in the same run;
-`value` is the chosen method's images/s.  Its error against the FP64 oracle
-is measured live on a sample of the instances (relative L1 of the normalised
-marginals per part, MAP agreement).
+The metric compares the chosen method (the x, rotation and y filters as
+banded-Toeplitz / circulant GEMMs on the matrix cores: the x filter bf16 x
+bf16, the rotation filter with data and taps split into bf16 hi + lo, an fp16
+intermediate, the y filter on fp16 data with split taps) with the plain FP32
+sweep (the drop-in) on the same plan and inputs, in the same run; `value` is
+the chosen method's images/s.  Its error against the FP64 oracle is measured
+live on a sample of the instances (relative L1 of the normalised marginals
+per part, MAP agreement).
 
 Prints one JSON line.  `roofline` is measured live over the timed steps with
-HIP events around each launch (on the stream the kernels run on): the
-dominant kernel's algorithmic HBM bytes per launch (the grids it must read
-and write) over its average launch time, against the published HBM peak;
-the issued MFMA flops of the Toeplitz GEMMs over the BF16 dense peak are
-reported beside it.  `cpu_baseline` times the oracle (the float32 CPU
+HIP events around each launch of the dominant kernel (on the stream the
+kernels run on): its algorithmic HBM bytes per launch (the grids it must read
+and write) over its average launch time, against the published HBM peak; a
+separately profiled pass gives the per-kernel breakdown, and the issued MFMA
+flops of the GEMMs over the BF16 dense peak are reported beside it.  `cpu_baseline` times the oracle (the float32 CPU
 restatement of the reference) on a bounded sample of the same instances, one
 instance per thread.
 """

@@ -287,7 +287,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU: instances shard by image across GPUs with no exchange, "
-              "and this build measures the single-GPU share (use --gpus 1)", file=sys.stderr)
+              "and this benchmark measures one GPU's share (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -1,7 +1,7 @@
 """Parity of the gfx950 sweep with the oracle (the CPU restatement of the
 reference), on the same seeded inputs.
 
-Bar: per part, the relative L1 distance of the
+Bar (the FP32 path's tolerance): per part, the relative L1 distance of the
 normalised marginals p = exp(b - LSE(b)) (computed in float64 from each side's
 float32 log marginals) is <= 1e-5 against the float32 oracle, and the MAP
 (rotation, y, x) of every part and scale is identical.  The distance to the

@@ -492,7 +492,10 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       }
       auto odbuf = [&](int k) { return gref(kBufScratch, od_off[k]); };
 
-      // upward levels: the edge of chain k whose child has height l
+      // upward levels, each message as late as its parent allows: the edge
+      // into a node at depth d runs at level max_height - d, so every message
+      // into the root (a one-part chain's too) runs in the last level and a
+      // level's messages share one destination kind in the person tree
       for (int k = 0; k < K; ++k) {
         const int len = (int)chains[k].size();
         for (int d = len; d >= 1; --d) {
@@ -529,7 +532,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
             mj.dst_add_unary = m.is_detect[p];
             mj.dst_key = up_key(p);
           }
-          up_jobs[t.height[c] - 1].push_back(mj);
+          up_jobs[t.max_height - t.depth[c]].push_back(mj);
         }
       }
       // root combine
@@ -559,7 +562,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         }
         B.roots.push_back(rj);
       }
-      // downward levels: the edge of chain k whose child has depth l
+      // downward levels, each message as late as its subtree allows: the edge
+      // into a node of height h runs at level max_depth - h (every leaf's in
+      // the last level); its parent's message ran in an earlier one
       for (int k = 0; k < K; ++k) {
         const int len = (int)chains[k].size();
         for (int d = 1; d <= len; ++d) {
@@ -588,7 +593,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
             mj.dst2 = kNone;
           }
           mj.map_slot = slot(c);
-          dn_jobs[d - 1].push_back(mj);
+          dn_jobs[t.max_depth - t.height[c]].push_back(mj);
         }
       }
       for (int p = 0; p < P; ++p) {

@@ -326,10 +326,9 @@ def test_profiling_classes_bracket_only_their_launches(gpu):
                 assert ms >= 0.0
         outs.append(marg.cpu().numpy())
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
-    # one column-pass launch per level and destination kind: up 1 and down 1
-    # each hold the head's message and the chains' messages, up 2 and down 2
-    # one kind each
-    assert prof["colpass"][1] == 6
+    # one column-pass launch per level and destination kind; the messages run
+    # as late as the tree allows, so each of the four levels holds one kind
+    assert prof["colpass"][1] == 4
     with pytest.raises(PsError):
         check(lib().ps_plan_set_profiling_classes(plan.handle, ctypes.c_uint(1 << len(Plan.KERNEL_CLASSES))),
               "ps_plan_set_profiling_classes")

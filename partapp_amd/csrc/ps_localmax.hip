@@ -45,6 +45,11 @@ __device__ __forceinline__ float okey_inv(unsigned k) {
   return __uint_as_float(u);
 }
 
+__global__ __launch_bounds__(256) void k_lmax_clear(int *__restrict__ cnt, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_lmax_mark(const float *__restrict__ grids, const LmGrid *__restrict__ gd,
                                                    unsigned long long *__restrict__ cand, int *__restrict__ count) {
   const LmGrid G = gd[blockIdx.y];
@@ -213,7 +218,10 @@ extern "C" int ps_find_local_max(const float *grids_dev, const long long *offset
   int *cnt = reinterpret_cast<int *>(base + o_cnt);
   unsigned long long *cand = reinterpret_cast<unsigned long long *>(base + o_cand);
   hipError_t e = hipMemcpyAsync(gdev, gd.data(), sizeof(LmGrid) * n_grids, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, sizeof(int) * n_grids, s);
+  if (e == hipSuccess) {  // counters cleared by a kernel (no memset node if a caller captures the stream)
+    hipLaunchKernelGGL(k_lmax_clear, dim3((n_grids + 255) / 256), dim3(256), 0, s, cnt, n_grids);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) {
     dim3 grid((unsigned)((max_n_el + 255) / 256), n_grids);
     hipLaunchKernelGGL(k_lmax_mark, grid, dim3(256), 0, s, grids_dev, gdev, cand, cnt);

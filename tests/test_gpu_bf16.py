@@ -177,6 +177,24 @@ def test_parts_without_detector(gpu, case, method):
         assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
 
 
+def test_fp16_widest_and_tallest_grids(gpu):
+    # the extremes of the uneven batch: 480 columns (8-wave row-pass blocks,
+    # 8 column strips) next to 480 rows, in one launch of each level
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = [synth_joints(120), synth_joints(121)]
+    model = person_model(ep, joints[0])
+    sizes = [(160, 480), (480, 168)]
+    us = [synth_unary(130 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(sizes)]
+    say("fp16 extremes", sizes)
+    outs, (levels, used) = run_method(model, us, joints, "bf16mix")
+    assert used == levels
+    for i in range(len(sizes)):
+        worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
+        say("  instance %d %s: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (i, sizes[i], worst, map_bad))
+        assert worst <= METHOD_L1["bf16mix"]
+        assert all(m[4] < 2 * METHOD_L1["bf16mix"] for m in map_bad), map_bad
+
+
 @pytest.mark.parametrize("method", ["bf16mix", "bf16x3"])
 def test_tiny_grids(gpu, method):
     # grids narrower and shorter than one MFMA tile (down to 1 x 1) in one

@@ -248,6 +248,28 @@ def test_method_fallback_for_wide_filters(gpu):
     assert worst <= BUDGET_L1 and not map_bad
 
 
+def test_width_limit_of_the_mfma_passes(gpu):
+    # 512 columns (8 full waves per row-pass block) is the widest grid the
+    # MFMA passes take; a batch with a wider grid runs every level in FP32,
+    # with the FP32 method's results bit for bit
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(140)
+    model = person_model(ep, joints)
+    u512 = synth_unary(141, 10, 1, 24, 96, 512)
+    outs, (levels, used) = run_method(model, [u512], [joints], "bf16mix")
+    assert used == levels == 4
+    worst, map_bad = errors_vs_f64(model, u512, joints, outs[0])
+    say("512 wide: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (worst, map_bad))
+    assert worst <= METHOD_L1["bf16mix"]
+    assert all(m[4] < 2 * METHOD_L1["bf16mix"] for m in map_bad), map_bad
+    u520 = synth_unary(142, 10, 1, 24, 96, 520)
+    outs_mix, (levels, used) = run_method(model, [u520], [joints], "bf16mix")
+    assert used == 0 and levels == 4
+    outs_fp32, _ = run_method(model, [u520], [joints], "fp32")
+    assert np.array_equal(outs_mix[0][0], outs_fp32[0][0])
+    assert np.array_equal(outs_mix[0][1], outs_fp32[0][1])
+
+
 def test_rows_per_block_knob_does_not_change_results(gpu):
     import torch
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)

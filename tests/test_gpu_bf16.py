@@ -270,6 +270,21 @@ def test_width_limit_of_the_mfma_passes(gpu):
     assert np.array_equal(outs_mix[0][1], outs_fp32[0][1])
 
 
+def test_other_rotation_counts_run_fp32(gpu):
+    # the MFMA passes hold 24 rotation slices per 32-row tile; with 16
+    # rotations every level runs the FP32 passes, bit for bit
+    ep = ExpParam(-180.0, 180.0, 16, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(150)
+    model = person_model(ep, joints)
+    u = synth_unary(151, 10, 1, 16, 72, 88)
+    outs_mix, (levels, used) = run_method(model, [u], [joints], "bf16mix")
+    assert used == 0 and levels == 4
+    outs_fp32, _ = run_method(model, [u], [joints], "fp32")
+    assert np.array_equal(outs_mix[0][0], outs_fp32[0][0])
+    worst, map_bad = errors_vs_f64(model, u, joints, outs_mix[0])
+    assert worst <= 1e-5 and not map_bad
+
+
 def test_rows_per_block_knob_does_not_change_results(gpu):
     import torch
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)

@@ -224,6 +224,42 @@ int ps_find_local_max(const float *grids_dev, const long long *offsets, const in
 int ps_ingest_scores(const float *score_grid_dev, int n, int grid_height, int grid_width, const double *Tig,
                      int interpolate, int height, int width, float *log_unary_dev, void *stream);
 
+/* Extra unaries: the additions findObjectImageRotJoints makes to the log
+ * unaries after ingestion and before the sweep (objectdetect_findrot.cpp:
+ * 896-948), in place on device grids, with the reference's arithmetic (float
+ * values; exp of the position prior in double; additions of a double term
+ * done in double and rounded once to float).  Asynchronous on `stream`; no
+ * allocation.
+ *
+ * setTorsoPosPrior (objectdetect_icps.cpp:137-191): root_unary_dev holds the
+ * root part's n_slices = S * R grids [n_slices][height][width]; params =
+ * (mu_x, mu_y, var_x, var_y) of torso_pos_prior.mat (getTorsoPosPriorParams,
+ * :32-45); adds log(max(1e-4, weight * exp(-(cx - x - mu_x)^2 / (2 var_x)) *
+ * exp(-(cy - y - mu_y)^2 / (2 var_y)))) with (cx, cy) = (width / 2, height / 2). */
+int ps_add_torso_pos_prior(float *root_unary_dev, int n_slices, int height, int width, const double params[4],
+                           float weight, void *stream);
+
+/* DPM scores for one part (part_unary_dev: [S][R][height][width]) from
+ * dpm_dev [n_dpm][height][width]:
+ *   PS_DPM_RAW        addLoadDPMScore's accumulation (objectdetect_icps.cpp:469-484)
+ *                     on raw scores v: += (v > 1e-4 ? weight * log(v) : log(1e-4));
+ *                     slice r of dpm when n_dpm == R, else slice 0;
+ *   PS_DPM_LOG        addDPMScore (:490-524) on log grids: += weight * g;
+ *                     slice r when n_dpm > 1 (then n_dpm >= R), else slice 0;
+ *                     weight must be > 0 (:501);
+ *   PS_DPM_LOG_OF_RAW computeLogGrid of raw scores (0 -> LOG_ZERO;
+ *                     multi_array_op.hpp:154-166) then PS_DPM_LOG, the DPM
+ *                     torso path (objectdetect_findrot.cpp:896-903).
+ * Loading the scoregrid MAT files (loadDPMScoreGrid, :550-575) is the
+ * caller's, e.g. through ps_mat_var. */
+enum { PS_DPM_RAW = 0, PS_DPM_LOG = 1, PS_DPM_LOG_OF_RAW = 2 };
+int ps_add_dpm_score(float *part_unary_dev, int S, int R, int height, int width, const float *dpm_dev, int n_dpm,
+                     float weight, int mode, void *stream);
+
+/* addExtraUnary (objectdetect_icps.cpp:526-548): log_unary_dev[i] +=
+ * weight * extra_dev[i] for n values (all parts, scales and rotations). */
+int ps_add_extra_unary(float *log_unary_dev, const float *extra_dev, long long n, float weight, void *stream);
+
 /* --------------------------------------------------------- MAT-file (level 5)
  * The on-disk formats around the path: joint_<child>_<parent>.mat
  * (objectdetect_learnparam.cpp:61-179), the cell_scoregrid / transform_Ti2 /

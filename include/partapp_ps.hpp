@@ -359,4 +359,98 @@ void computeRotJointMarginal(const ExpParamT &exp_param, Grid3 &log_prob_child, 
   check(st, "ps_message");
 }
 
+// ------------------------------------------------------------ extra unaries
+// The additions findObjectImageRotJoints makes to the log unaries before
+// computeRootPosteriorRot (objectdetect_findrot.cpp:896-948), with the
+// reference's signatures, on host grids (each call uploads the grids it
+// changes, adds on the GPU and downloads them).  On device unaries use
+// ps_add_torso_pos_prior / ps_add_dpm_score / ps_add_extra_unary directly.
+
+namespace detail {
+// f(device grid of part p, scale s) for every (p, s) in `parts`
+template <class Grid3, class F>
+void on_device(std::vector<std::vector<Grid3>> &grids, const std::vector<int> &parts, F f) {
+  for (int p : parts)
+    for (auto &g : grids[p]) {
+      const size_t n = g.num_elements();
+      DeviceBuffer d(n * sizeof(float));
+      check_hip(hipMemcpy(d.p, g.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+      f(d.f(), (int)g.shape()[0], (int)g.shape()[1], (int)g.shape()[2], p);
+      check_hip(hipMemcpy(g.data(), d.p, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+}
+template <class Grid2>
+size_t slices_bytes(const std::vector<Grid2> &g) {
+  if (g.empty()) throw std::invalid_argument("no DPM score grids");
+  return g.size() * g[0].num_elements() * sizeof(float);
+}
+template <class Grid2>
+void upload_slices(const std::vector<Grid2> &g, const DeviceBuffer &d) {
+  const size_t hw = g[0].num_elements();
+  for (size_t i = 0; i < g.size(); ++i)
+    check_hip(hipMemcpy(d.f() + i * hw, g[i].data(), hw * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+}
+}  // namespace detail
+
+// setTorsoPosPrior (objectdetect_icps.cpp:137-191); MatT: pos_prior_params(0, j)
+template <class PartAppT, class Grid3, class MatT>
+void setTorsoPosPrior(const PartAppT &part_app, std::vector<std::vector<Grid3>> &log_part_detections,
+                      const MatT &pos_prior_params, int rootpart_idx) {
+  const double params[4] = {pos_prior_params(0, 0), pos_prior_params(0, 1), pos_prior_params(0, 2),
+                            pos_prior_params(0, 3)};
+  const float weight = part_app.m_exp_param.torso_pos_prior_weight();
+  detail::on_device(log_part_detections, {rootpart_idx}, [&](float *g, int R, int H, int W, int) {
+    check(ps_add_torso_pos_prior(g, R, H, W, params, weight, nullptr), "ps_add_torso_pos_prior");
+  });
+}
+
+// addDPMScore (objectdetect_icps.cpp:490-524): log grids, one or one per rotation
+template <class PartAppT, class Grid3, class Grid2>
+void addDPMScore(const PartAppT &part_app, std::vector<std::vector<Grid3>> &log_part_detections,
+                 std::vector<Grid2> log_dpmPriorGrid, int pidx, float dpm_weight) {
+  (void)part_app;
+  DeviceBuffer d(detail::slices_bytes(log_dpmPriorGrid));
+  detail::upload_slices(log_dpmPriorGrid, d);
+  const int n = (int)log_dpmPriorGrid.size();
+  detail::on_device(log_part_detections, {pidx}, [&](float *g, int R, int H, int W, int) {
+    check(ps_add_dpm_score(g, 1, R, H, W, d.f(), n, dpm_weight, PS_DPM_LOG, nullptr), "ps_add_dpm_score");
+  });
+}
+
+// addLoadDPMScore (objectdetect_icps.cpp:445-488) with the scoregrid slices
+// already loaded (loadDPMScoreGrid, :550-575): raw scores for every detected
+// part, or for pidx_only when it names one
+template <class PartAppT, class Grid3, class Grid2>
+void addDPMScoreRaw(const PartAppT &part_app, std::vector<std::vector<Grid3>> &log_part_detections,
+                    const std::vector<Grid2> &dpmScoreGrid, float dpm_weight, int pidx_only = -1) {
+  const int P = part_app.m_part_conf.part_size();
+  std::vector<int> parts;
+  for (int p = 0; p < P; ++p)
+    if (!(pidx_only > -1 && pidx_only < P && pidx_only != p) && part_app.m_part_conf.part(p).is_detect())
+      parts.push_back(p);
+  DeviceBuffer d(detail::slices_bytes(dpmScoreGrid));
+  detail::upload_slices(dpmScoreGrid, d);
+  const int n = (int)dpmScoreGrid.size();
+  detail::on_device(log_part_detections, parts, [&](float *g, int R, int H, int W, int) {
+    check(ps_add_dpm_score(g, 1, R, H, W, d.f(), n, dpm_weight, PS_DPM_RAW, nullptr), "ps_add_dpm_score");
+  });
+}
+
+// addExtraUnary (objectdetect_icps.cpp:526-548)
+template <class PartAppT, class Grid3>
+void addExtraUnary(const PartAppT &part_app, std::vector<std::vector<Grid3>> &log_part_detections,
+                   const std::vector<std::vector<Grid3>> &log_extra_unary_scores, float weight) {
+  const int P = part_app.m_part_conf.part_size();
+  std::vector<int> parts(P);
+  for (int p = 0; p < P; ++p) parts[p] = p;
+  std::vector<size_t> next(P, 0);
+  detail::on_device(log_part_detections, parts, [&](float *g, int R, int H, int W, int p) {
+    const auto &e = log_extra_unary_scores[p][next[p]++];
+    const size_t n = (size_t)R * H * W;
+    DeviceBuffer d(n * sizeof(float));
+    check_hip(hipMemcpy(d.p, e.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    check(ps_add_extra_unary(g, d.f(), (long long)n, weight, nullptr), "ps_add_extra_unary");
+  });
+}
+
 }  // namespace partapp_ps

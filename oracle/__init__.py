@@ -309,3 +309,49 @@ def ingest_scores(score_grid, Tig, interpolate, H, W):
     out = np.empty((n, H, W), dtype=np.float32)
     L.orc_ingest_scores_f32(_ptr(g), n, gh, gw, _ptr(T), int(interpolate), H, W, _ptr(out))
     return out
+
+
+# ------------------------------------------------------- extra unaries (f3)
+
+def torso_pos_prior(grids, params, weight):
+    """setTorsoPosPrior on a copy of grids [..][H][W] (float32)."""
+    g = np.array(grids, dtype=np.float32, copy=True, order="C")
+    H, W = g.shape[-2:]
+    p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).ravel()[:4])
+    _chk(lib().orc_torso_pos_prior_f32(_ptr(g), g.size // (H * W), H, W, _ptr(p), ctypes.c_float(weight)),
+         "torso_pos_prior")
+    return g
+
+
+def add_dpm_raw(grids, dpm, weight):
+    """addLoadDPMScore's accumulation on a copy of one part's [S][R][H][W]."""
+    g = np.array(grids, dtype=np.float32, copy=True, order="C")
+    d = np.ascontiguousarray(dpm, dtype=np.float32).reshape(-1, *g.shape[-2:])
+    S, R, H, W = g.shape
+    _chk(lib().orc_add_dpm_raw_f32(_ptr(g), S, R, H, W, _ptr(d), d.shape[0], ctypes.c_float(weight)), "add_dpm_raw")
+    return g
+
+
+def add_dpm_log(grids, log_dpm, weight):
+    """addDPMScore on a copy of one part's [S][R][H][W]."""
+    g = np.array(grids, dtype=np.float32, copy=True, order="C")
+    d = np.ascontiguousarray(log_dpm, dtype=np.float32).reshape(-1, *g.shape[-2:])
+    S, R, H, W = g.shape
+    _chk(lib().orc_add_dpm_log_f32(_ptr(g), S, R, H, W, _ptr(d), d.shape[0], ctypes.c_float(weight)), "add_dpm_log")
+    return g
+
+
+def log_grid(grid):
+    """computeLogGrid on a copy (0 -> LOG_ZERO)."""
+    g = np.array(grid, dtype=np.float32, copy=True, order="C")
+    _chk(lib().orc_log_grid_f32(_ptr(g), ctypes.c_longlong(g.size)), "log_grid")
+    return g
+
+
+def add_extra_unary(grids, extra, weight):
+    g = np.array(grids, dtype=np.float32, copy=True, order="C")
+    e = np.ascontiguousarray(extra, dtype=np.float32)
+    assert e.size == g.size
+    _chk(lib().orc_add_extra_unary_f32(_ptr(g), _ptr(e), ctypes.c_longlong(g.size), ctypes.c_float(weight)),
+         "add_extra_unary")
+    return g

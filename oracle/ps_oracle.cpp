@@ -896,6 +896,92 @@ int orc_ingest_scores_f32(const float *score_grid, int n, int gh, int gw, const 
   return ORC_OK;
 }
 
+// ------------------------------------------- extra unaries (f3), float only
+// The reference evaluates these with `using namespace std` (objectdetect_icps.cpp:28):
+// log(float) is the float overload, exp(double) and pow(float, int) work in
+// double, and `float += double` adds in double and rounds once to float.
+
+// setTorsoPosPrior (objectdetect_icps.cpp:137-191)
+int orc_torso_pos_prior_f32(float *grids, int SR, int H, int W, const double *params, float weight) {
+  if (!grids || !params || SR < 1 || H < 1 || W < 1) return ORC_ERR_ARG;
+  float mu_x = (float)params[0];
+  float mu_y = (float)params[1];
+  float var_x = (float)params[2];
+  float var_y = (float)params[3];
+  float img_c_x = 0.5 * W;
+  float img_c_y = 0.5 * H;
+  float sigma_weight = 1.0;
+  float var_weight = sigma_weight * sigma_weight;
+  var_x *= var_weight;
+  var_y *= var_weight;
+  std::vector<double> pred_pos((size_t)H * W);
+  for (int iy = 0; iy < H; ++iy)
+    for (int ix = 0; ix < W; ++ix) {
+      float ix_rel = img_c_x - ix;
+      float iy_rel = img_c_y - iy;
+      float score_x = std::exp(-0.5 * std::pow(ix_rel - mu_x, 2) / var_x);
+      float score_y = std::exp(-0.5 * std::pow(iy_rel - mu_y, 2) / var_y);
+      float score = weight * score_x * score_y;
+      if (score < 1e-4) score = 1e-4;
+      score = std::log(score);
+      pred_pos[(size_t)iy * W + ix] = score;
+    }
+  for (int k = 0; k < SR; ++k)
+    for (size_t i = 0; i < (size_t)H * W; ++i) {
+      float &g = grids[(size_t)k * H * W + i];
+      g += pred_pos[i];
+    }
+  return ORC_OK;
+}
+
+// addLoadDPMScore's accumulation for one part (objectdetect_icps.cpp:469-484):
+// dpm [n_rot_dpm][H][W] raw scores, slice rotidx if n_rot_dpm == R else 0
+int orc_add_dpm_raw_f32(float *grids, int S, int R, int H, int W, const float *dpm, int n_rot_dpm, float weight) {
+  if (!grids || !dpm || S < 1 || R < 1 || H < 1 || W < 1 || n_rot_dpm < 1) return ORC_ERR_ARG;
+  for (int s = 0; s < S; ++s)
+    for (int r = 0; r < R; ++r) {
+      int rd = (n_rot_dpm == R ? r : 0);
+      for (size_t i = 0; i < (size_t)H * W; ++i) {
+        float val = dpm[(size_t)rd * H * W + i];
+        float &g = grids[((size_t)s * R + r) * H * W + i];
+        g += (val > 1e-4 ? weight * std::log(val) : std::log(1e-4));
+      }
+    }
+  return ORC_OK;
+}
+
+// addDPMScore (objectdetect_icps.cpp:490-524): log grids [n_log][H][W],
+// slice ridx when there is more than one, else slice 0
+int orc_add_dpm_log_f32(float *grids, int S, int R, int H, int W, const float *log_dpm, int n_log, float weight) {
+  if (!grids || !log_dpm || S < 1 || R < 1 || H < 1 || W < 1 || n_log < 1 || !(weight > 0)) return ORC_ERR_ARG;
+  bool bRot = n_log > 1;
+  if (bRot && n_log < R) return ORC_ERR_ARG;
+  for (int s = 0; s < S; ++s)
+    for (int r = 0; r < R; ++r)
+      for (size_t i = 0; i < (size_t)H * W; ++i) {
+        float &g = grids[((size_t)s * R + r) * H * W + i];
+        g += weight * log_dpm[(size_t)(bRot ? r : 0) * H * W + i];
+      }
+  return ORC_OK;
+}
+
+// computeLogGrid (multi_array_op.hpp:154-166)
+int orc_log_grid_f32(float *grid, long long n) {
+  if (!grid || n < 0) return ORC_ERR_ARG;
+  for (long long i = 0; i < n; ++i) {
+    if (!(grid[i] >= 0)) return ORC_ERR_ARG;  // the reference asserts data >= 0
+    grid[i] = grid[i] == 0 ? (float)LOG_ZERO : std::log(grid[i]);
+  }
+  return ORC_OK;
+}
+
+// addExtraUnary (objectdetect_icps.cpp:526-548)
+int orc_add_extra_unary_f32(float *grids, const float *extra, long long n, float weight) {
+  if (!grids || !extra || n < 0) return ORC_ERR_ARG;
+  for (long long i = 0; i < n; ++i) grids[i] += weight * extra[i];
+  return ORC_OK;
+}
+
 int orc_message_plan(const orc_exp_param *ep, int R, int H, int W, const double *offset_c, const double *offset_p,
                      double rot_mean, double rot_sigma, double scale, int *delta, int *fwd_ox, int *fwd_oy,
                      int *inv_ox, int *inv_oy, int *uniform, int *rot_len, double *rot_taps, int *kx, int *ky) {

@@ -113,6 +113,21 @@ int orc_transform_bilinear_f32(const float *in, int in_h, int in_w, float *out, 
 int orc_ingest_scores_f32(const float *score_grid, int n, int gh, int gw, const double *Tig, int interpolate, int H,
                           int W, float *out);
 
+/* Extra unaries added to the log unaries before the sweep
+ * (objectdetect_findrot.cpp:896-948), float arithmetic as the reference:
+ *   setTorsoPosPrior (objectdetect_icps.cpp:137-191) on the root part's
+ *     [SR][H][W] slices, params = mu_x, mu_y, var_x, var_y;
+ *   addLoadDPMScore's accumulation (:469-484) on one part's [S][R][H][W] from
+ *     raw scores dpm [n_rot_dpm][H][W];
+ *   addDPMScore (:490-524) from log grids [n_log][H][W];
+ *   computeLogGrid (multi_array_op.hpp:154-166);
+ *   addExtraUnary (:526-548) over n values. */
+int orc_torso_pos_prior_f32(float *grids, int SR, int H, int W, const double *params, float weight);
+int orc_add_dpm_raw_f32(float *grids, int S, int R, int H, int W, const float *dpm, int n_rot_dpm, float weight);
+int orc_add_dpm_log_f32(float *grids, int S, int R, int H, int W, const float *log_dpm, int n_log, float weight);
+int orc_log_grid_f32(float *grid, long long n);
+int orc_add_extra_unary_f32(float *grids, const float *extra, long long n, float weight);
+
 /* The integer plan of one message, evaluated pixel by pixel with the
  * reference's own double arithmetic (objectdetect_findrot.cpp:316-359,437-448;
  * multi_array_transform.hpp:193-216).  For every rotation r the forward warp

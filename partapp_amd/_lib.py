@@ -121,6 +121,12 @@ SIGNATURES = {
                                          ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_ingest_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ps_add_torso_pos_prior": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_double_p,
+                                              ctypes.c_float, ctypes.c_void_p]),
+    "ps_add_dpm_score": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]),
+    "ps_add_extra_unary": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_float,
+                                          ctypes.c_void_p]),
     "ps_mat_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
     "ps_mat_close": (None, [ctypes.c_void_p]),
     "ps_mat_var": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_longlong, c_int_p, c_int_p,

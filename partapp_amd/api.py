@@ -393,6 +393,56 @@ def ingest_scores(score_grid, Tig, interpolate, height, width, stream=None):
     return out
 
 
+# ------------------------------------------------------------ extra unaries
+
+DPM_RAW, DPM_LOG, DPM_LOG_OF_RAW = 0, 1, 2
+
+
+def _grid_dims(t, what):
+    assert t.is_cuda and t.dtype.is_floating_point and t.is_contiguous(), what + ": a contiguous CUDA float32 tensor"
+    return t
+
+
+def set_torso_pos_prior(root_unary, pos_prior_params, weight, stream=None):
+    """setTorsoPosPrior (objectdetect_icps.cpp:137-191) in place on the root
+    part's log unaries [S][R][H][W] (or [n][H][W]); pos_prior_params =
+    (mu_x, mu_y, var_x, var_y), the `params` row of torso_pos_prior.mat."""
+    _grid_dims(root_unary, "root_unary")
+    H, W = root_unary.shape[-2:]
+    n = root_unary.numel() // (H * W)
+    pp = (ctypes.c_double * 4)(*[float(v) for v in np.asarray(pos_prior_params, dtype=np.float64).ravel()[:4]])
+    check(lib().ps_add_torso_pos_prior(root_unary.data_ptr(), n, H, W, pp, float(weight), _stream_handle(stream)),
+          "ps_add_torso_pos_prior")
+    return root_unary
+
+
+def add_dpm_score(part_unary, dpm, weight, mode=DPM_RAW, stream=None):
+    """DPM scores into one part's log unaries [S][R][H][W] from dpm [n][H][W]
+    (CUDA float32): mode DPM_RAW = addLoadDPMScore's accumulation
+    (objectdetect_icps.cpp:469-484), DPM_LOG = addDPMScore (:490-524) on log
+    grids, DPM_LOG_OF_RAW = computeLogGrid then addDPMScore (the DPM torso
+    path, objectdetect_findrot.cpp:896-903)."""
+    _grid_dims(part_unary, "part_unary")
+    _grid_dims(dpm, "dpm")
+    S, R, H, W = part_unary.shape
+    if dpm.dim() == 2:
+        dpm = dpm.unsqueeze(0)
+    assert tuple(dpm.shape[1:]) == (H, W)
+    check(lib().ps_add_dpm_score(part_unary.data_ptr(), S, R, H, W, dpm.data_ptr(), dpm.shape[0], float(weight),
+                                 int(mode), _stream_handle(stream)), "ps_add_dpm_score")
+    return part_unary
+
+
+def add_extra_unary(log_unary, extra, weight, stream=None):
+    """addExtraUnary (objectdetect_icps.cpp:526-548): log_unary += weight * extra."""
+    _grid_dims(log_unary, "log_unary")
+    _grid_dims(extra, "extra")
+    assert extra.numel() == log_unary.numel()
+    check(lib().ps_add_extra_unary(log_unary.data_ptr(), extra.data_ptr(), log_unary.numel(), float(weight),
+                                   _stream_handle(stream)), "ps_add_extra_unary")
+    return log_unary
+
+
 # ----------------------------------------------------------- synthetic data
 
 PERSON_ROOT = 4
@@ -440,6 +490,12 @@ def config_exp_param(cfg: int) -> ExpParam:
     if cfg in (0, 1):
         return ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
     return ExpParam(-180.0, 180.0, 24, 0.7, 1.3, 3, 0.0)
+
+
+def uneven_sizes(n, seed=20131201):
+    """Sizes of the uneven batch: H and W uniform in {160, 168, ..., 480}."""
+    rng = np.random.default_rng(seed)
+    return [(8 * int(rng.integers(20, 61)), 8 * int(rng.integers(20, 61))) for _ in range(n)]
 
 
 # ------------------------------------------------------------ MAT-file (level 5)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "ps_internal.hpp"
+#include "ps_message.hpp"
 #include "ps_plan.hpp"
 
 namespace ps {
@@ -24,6 +25,7 @@ hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys
 hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
 size_t rowpass_lds_bytes(int R, int W, int kx);
 hipError_t rowpass_set_lds_limit(size_t bytes);
+hipError_t colpass_set_lds_limit(size_t bytes);
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                           int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
 size_t colpass_lds_bytes(int ky);
@@ -746,10 +748,14 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     delete pl;
     return hip_error(e, "plan upload");
   }
-  size_t max_lds = 0;
-  for (auto &L : pl->up) max_lds = std::max(max_lds, L.lds);
-  for (auto &L : pl->down) max_lds = std::max(max_lds, L.lds);
+  size_t max_lds = 0, max_col_lds = 0;
+  for (auto *lv : {&pl->up, &pl->down})
+    for (auto &L : *lv) {
+      max_lds = std::max(max_lds, L.lds);
+      max_col_lds = std::max(max_col_lds, L.col_lds);
+    }
   e = rowpass_set_lds_limit(max_lds);
+  if (e == hipSuccess) e = colpass_set_lds_limit(max_col_lds);
   size_t mrow = 0, mcol = 0;
   for (auto *lv : {&pl->up, &pl->down})
     for (auto &L : *lv)
@@ -1027,11 +1033,16 @@ extern "C" int ps_marginals(const ps_model *model, const ps_instance *instances,
   return st;
 }
 
-extern "C" int ps_message(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
-                          const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
-                          double rot_sigma, double scale, int is_sparse, void *stream) {
-  if (!model || !child_dev || !parent_dev || !offset_c || !offset_p || !C)
-    return set_error(PS_ERR_INVALID_ARG, "null argument");
+namespace ps {
+
+// One message (computeRotJointMarginal) laid out in one block of device
+// memory: its job and max-job records, taps, shifts, the two keys, then the
+// row-pass output grid and, for an off-diagonal covariance, the eigenbasis
+// grids.
+int message_prepare(const ps_model *model, int height, int width, const double offset_c[2], const double offset_p[2],
+                    const double C[4], double rot_mean, double rot_sigma, double scale, int is_sparse,
+                    MessageSetup &ms) {
+  if (!model || !offset_c || !offset_p || !C) return set_error(PS_ERR_INVALID_ARG, "null argument");
   if (height < 1 || width < 1) return set_error(PS_ERR_INVALID_ARG, "sizes must be >= 1");
   const Model &m = model->m;
   const int R = (int)m.ep.num_rotation_steps;
@@ -1047,8 +1058,14 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   mj.H = height;
   mj.W = width;
   B.add_message(mp, mj);
-  const size_t lds = rowpass_lds_bytes(R, width, mj.kx);
-  if (lds > kLdsLimit || (!mp.od && colpass_lds_bytes(mp.ky) > kLdsLimit))
+  ms.R = R;
+  ms.H = height;
+  ms.W = width;
+  ms.od = mp.od != 0;
+  ms.od_cells = mp.od ? (long long)R * mp.th * mp.tw : 0;
+  ms.lds = rowpass_lds_bytes(R, width, mj.kx);
+  ms.col_lds = mp.od ? 0 : colpass_lds_bytes(mp.ky);
+  if (ms.lds > kLdsLimit || ms.col_lds > kLdsLimit)
     return set_error(PS_ERR_UNSUPPORTED, "message needs more LDS than one workgroup has");
   mj.src_kind = kSrcUnary;  // a plain grid, read through the (empty) mask path
   mj.src = gref(kBufUnary, 0);
@@ -1060,6 +1077,7 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   mj.dst2 = kNone;
   const long long n64 = ((long long)R * height * t_pitch(width) + 63) & ~63ll;
   mj.od_buf = mp.od ? gref(kBufScratch, n64) : kNone;
+  ms.epi = epi_kind(mj);
   MaxJob xj;
   std::memset(&xj, 0, sizeof(xj));
   xj.H = height;
@@ -1069,47 +1087,69 @@ extern "C" int ps_message(const ps_model *model, const float *child_dev, float *
   xj.src = gref(kBufUnary, 0);
 
   auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
-  size_t o_job = 0, o_max = align(sizeof(MsgJob)), o_taps = align(o_max + sizeof(MaxJob));
-  size_t o_sh = align(o_taps + B.taps.size() * sizeof(float));
-  size_t o_key = align(o_sh + B.shifts.size() * sizeof(int));
-  size_t o_mk = align(o_key + sizeof(unsigned));
-  size_t o_T = align(o_mk + sizeof(unsigned long long));
-  size_t total = o_T + sizeof(float) * (size_t)(n64 + (mp.od ? od_scratch_floats(R, mp.th, mp.tw) : 0));
-  void *mem = nullptr;
-  hipError_t e = hipMalloc(&mem, total);
-  if (e != hipSuccess) return set_error(PS_ERR_OUT_OF_MEMORY, "message scratch");
-  std::vector<char> host(o_T, 0);
-  std::memcpy(host.data() + o_job, &mj, sizeof(mj));
-  std::memcpy(host.data() + o_max, &xj, sizeof(xj));
-  std::memcpy(host.data() + o_taps, B.taps.data(), B.taps.size() * sizeof(float));
-  std::memcpy(host.data() + o_sh, B.shifts.data(), B.shifts.size() * sizeof(int));
-  char *base = static_cast<char *>(mem);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  e = hipMemcpyAsync(mem, host.data(), o_T, hipMemcpyHostToDevice, s);
+  ms.o_job = 0;
+  ms.o_max = align(sizeof(MsgJob));
+  ms.o_taps = align(ms.o_max + sizeof(MaxJob));
+  ms.o_sh = align(ms.o_taps + B.taps.size() * sizeof(float));
+  ms.o_key = align(ms.o_sh + B.shifts.size() * sizeof(int));
+  ms.o_mk = align(ms.o_key + sizeof(unsigned));
+  ms.o_T = align(ms.o_mk + sizeof(unsigned long long));
+  ms.total = ms.o_T + sizeof(float) * (size_t)(n64 + (mp.od ? od_scratch_floats(R, mp.th, mp.tw) : 0));
+  ms.host.assign(ms.o_T, 0);
+  std::memcpy(ms.host.data() + ms.o_job, &mj, sizeof(mj));
+  std::memcpy(ms.host.data() + ms.o_max, &xj, sizeof(xj));
+  std::memcpy(ms.host.data() + ms.o_taps, B.taps.data(), B.taps.size() * sizeof(float));
+  std::memcpy(ms.host.data() + ms.o_sh, B.shifts.data(), B.shifts.size() * sizeof(int));
+  return PS_OK;
+}
+
+// Upload the tables into `mem` (at least ms.total bytes of device memory),
+// run the message and wait for it.  The two keys are cleared by a kernel.
+int message_enqueue(const MessageSetup &ms, char *mem, const float *child_dev, float *parent_dev, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(mem, ms.host.data(), ms.o_T, hipMemcpyHostToDevice, s);
   Bases b;
   b.unary = child_dev;
   b.marginal = parent_dev;
-  b.scratch = reinterpret_cast<float *>(base + o_T);
+  b.scratch = reinterpret_cast<float *>(mem + ms.o_T);
   b.root = nullptr;
-  b.max_keys = reinterpret_cast<unsigned *>(base + o_key);
-  b.map_keys = reinterpret_cast<unsigned long long *>(base + o_mk);
-  const MsgJob *jd = reinterpret_cast<const MsgJob *>(base + o_job);
-  const float *td = reinterpret_cast<const float *>(base + o_taps);
-  const int *sd = reinterpret_cast<const int *>(base + o_sh);
-  if (e == hipSuccess) e = rowpass_set_lds_limit(lds);
+  b.max_keys = reinterpret_cast<unsigned *>(mem + ms.o_key);
+  b.map_keys = reinterpret_cast<unsigned long long *>(mem + ms.o_mk);
+  const MsgJob *jd = reinterpret_cast<const MsgJob *>(mem + ms.o_job);
+  const float *td = reinterpret_cast<const float *>(mem + ms.o_taps);
+  const int *sd = reinterpret_cast<const int *>(mem + ms.o_sh);
+  if (e == hipSuccess) e = rowpass_set_lds_limit(ms.lds);
+  if (e == hipSuccess && !ms.od) e = colpass_set_lds_limit(ms.col_lds);
   if (e == hipSuccess) e = launch_reset(b.max_keys, 1, b.map_keys, 1, s);
-  if (e == hipSuccess) e = launch_max(b, reinterpret_cast<const MaxJob *>(base + o_max), 1, R, height * width, s);
-  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, R, height, width, lds, s);
+  if (e == hipSuccess)
+    e = launch_max(b, reinterpret_cast<const MaxJob *>(mem + ms.o_max), 1, ms.R, ms.H * ms.W, s);
+  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, ms.R, ms.H, ms.W, ms.lds, s);
   if (e == hipSuccess) {
-    if (mp.od)
-      e = launch_offdiag(b, jd, 1, td, sd, R, (long long)R * mp.th * mp.tw, (long long)height * width, s);
+    if (ms.od)
+      e = launch_offdiag(b, jd, 1, td, sd, ms.R, ms.od_cells, (long long)ms.H * ms.W, s);
     else
-      e = launch_colpass(b, jd, 1, td, sd, R, epi_kind(mj), width, colpass_lds_bytes(mp.ky), s);
+      e = launch_colpass(b, jd, 1, td, sd, ms.R, ms.epi, ms.W, ms.col_lds, s);
   }
+  // the caller's tables live in pageable host memory and the block may be
+  // reused right after: the call is synchronous
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(mem);
   if (e != hipSuccess) return hip_error(e, "message");
   return PS_OK;
+}
+
+}  // namespace ps
+
+extern "C" int ps_message(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
+                          const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
+                          double rot_sigma, double scale, int is_sparse, void *stream) {
+  if (!child_dev || !parent_dev) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  MessageSetup ms;
+  int st = message_prepare(model, height, width, offset_c, offset_p, C, rot_mean, rot_sigma, scale, is_sparse, ms);
+  if (st != PS_OK) return st;
+  void *mem = nullptr;
+  if (hipMalloc(&mem, ms.total) != hipSuccess) return set_error(PS_ERR_OUT_OF_MEMORY, "message scratch");
+  st = message_enqueue(ms, static_cast<char *>(mem), child_dev, parent_dev, static_cast<hipStream_t>(stream));
+  (void)hipFree(mem);
+  return st;
 }
 
 // --------------------------------------------------------------- utilities

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -169,5 +170,120 @@ extern "C" int ps_ingest_scores(const float *score_grid_dev, int n, int grid_hei
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(ws);
   if (e != hipSuccess) return set_error(PS_ERR_HIP, std::string("ingest: ") + hipGetErrorString(e));
+  return PS_OK;
+}
+
+// ------------------------------------------------------------- extra unaries
+// The additions findObjectImageRotJoints makes to the log unaries before the
+// sweep (objectdetect_findrot.cpp:896-948), in place on device grids, with
+// the reference's float / double mix (objectdetect_icps.cpp is compiled with
+// `using namespace std`: log(float) is the float log, exp and pow(float, int)
+// work in double, and `float += double` adds in double and rounds once).
+
+namespace ps {
+namespace {
+
+// setTorsoPosPrior (objectdetect_icps.cpp:137-191): one thread per pixel
+// forms the clipped log prior once, then adds it to every slice.
+__global__ __launch_bounds__(kIngThreads) void k_torso_prior(float *g, int n_slices, int H, int W, float mu_x,
+                                                             float mu_y, float var_x, float var_y, float weight) {
+  const long long hw = (long long)H * W;
+  const long long i = (long long)blockIdx.x * kIngThreads + threadIdx.x;
+  if (i >= hw) return;
+  const int iy = (int)(i / W), ix = (int)(i - (long long)iy * W);
+  const float img_c_x = (float)(0.5 * W), img_c_y = (float)(0.5 * H);
+  const float ix_rel = img_c_x - (float)ix;
+  const float iy_rel = img_c_y - (float)iy;
+  // pow(d, 2) of a float d is exact in double
+  const double dx = (double)(ix_rel - mu_x), dy = (double)(iy_rel - mu_y);
+  const float score_x = (float)exp(-0.5 * (dx * dx) / (double)var_x);
+  const float score_y = (float)exp(-0.5 * (dy * dy) / (double)var_y);
+  float score = weight * score_x * score_y;
+  if ((double)score < 1e-4) score = 1e-4f;
+  const double pred = (double)logf(score);
+  for (int k = 0; k < n_slices; ++k) {
+    float *p = g + (long long)k * hw + i;
+    *p = (float)((double)*p + pred);
+  }
+}
+
+// addLoadDPMScore's accumulation (:469-484, raw scores), addDPMScore
+// (:490-524, log grids) and the torso path's computeLogGrid + addDPMScore
+// (objectdetect_findrot.cpp:896-903): one thread per pixel, all slices
+__global__ __launch_bounds__(kIngThreads) void k_dpm_add(float *g, int S, int R, int H, int W, const float *dpm,
+                                                         int n_dpm, float weight, int mode, double log_clip) {
+  const long long hw = (long long)H * W;
+  const long long i = (long long)blockIdx.x * kIngThreads + threadIdx.x;
+  if (i >= hw) return;
+  for (int s = 0; s < S; ++s)
+    for (int r = 0; r < R; ++r) {
+      float *p = g + ((long long)s * R + r) * hw + i;
+      if (mode == PS_DPM_RAW) {
+        const float val = dpm[(long long)(n_dpm == R ? r : 0) * hw + i];
+        const double add = (double)val > 1e-4 ? (double)(weight * logf(val)) : log_clip;
+        *p = (float)((double)*p + add);
+      } else {
+        float lv = dpm[(long long)(n_dpm > 1 ? r : 0) * hw + i];
+        if (mode == PS_DPM_LOG_OF_RAW) lv = lv == 0.0f ? kLogZero : logf(lv);
+        *p = *p + weight * lv;
+      }
+    }
+}
+
+// addExtraUnary (:526-548)
+__global__ __launch_bounds__(kIngThreads) void k_add_scaled(float *g, const float *e, long long n, float weight) {
+  for (long long i = (long long)blockIdx.x * kIngThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kIngThreads)
+    g[i] = g[i] + weight * e[i];
+}
+
+unsigned pixel_blocks(long long hw) { return (unsigned)((hw + kIngThreads - 1) / kIngThreads); }
+
+}  // namespace
+}  // namespace ps
+
+extern "C" int ps_add_torso_pos_prior(float *root_unary_dev, int n_slices, int height, int width,
+                                      const double params[4], float weight, void *stream) {
+  if (!root_unary_dev || !params) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  if (n_slices < 1 || height < 1 || width < 1) return set_error(PS_ERR_INVALID_ARG, "counts and sizes must be >= 1");
+  if ((long long)height * width >= (1ll << 40)) return set_error(PS_ERR_UNSUPPORTED, "grid too large");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_torso_prior, dim3(pixel_blocks((long long)height * width)), dim3(kIngThreads), 0, st,
+                     root_unary_dev, n_slices, height, width, (float)params[0], (float)params[1], (float)params[2],
+                     (float)params[3], weight);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(PS_ERR_HIP, std::string("torso prior: ") + hipGetErrorString(e));
+  return PS_OK;
+}
+
+extern "C" int ps_add_dpm_score(float *part_unary_dev, int S, int R, int height, int width, const float *dpm_dev,
+                                int n_dpm, float weight, int mode, void *stream) {
+  if (!part_unary_dev || !dpm_dev) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  if (S < 1 || R < 1 || height < 1 || width < 1 || n_dpm < 1)
+    return set_error(PS_ERR_INVALID_ARG, "counts and sizes must be >= 1");
+  if (mode != PS_DPM_RAW && mode != PS_DPM_LOG && mode != PS_DPM_LOG_OF_RAW)
+    return set_error(PS_ERR_INVALID_ARG, "unknown DPM mode " + std::to_string(mode));
+  if (mode != PS_DPM_RAW) {
+    if (!(weight > 0)) return set_error(PS_ERR_INVALID_ARG, "dpm_weight must be > 0 (objectdetect_icps.cpp:501)");
+    if (n_dpm > 1 && n_dpm < R) return set_error(PS_ERR_INVALID_ARG, "a rotation-dependent DPM prior needs R grids");
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_dpm_add, dim3(pixel_blocks((long long)height * width)), dim3(kIngThreads), 0, st,
+                     part_unary_dev, S, R, height, width, dpm_dev, n_dpm, weight, mode, std::log(1e-4));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(PS_ERR_HIP, std::string("dpm score: ") + hipGetErrorString(e));
+  return PS_OK;
+}
+
+extern "C" int ps_add_extra_unary(float *log_unary_dev, const float *extra_dev, long long n, float weight,
+                                  void *stream) {
+  if (!log_unary_dev || !extra_dev) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  if (n < 0) return set_error(PS_ERR_INVALID_ARG, "n must be >= 0");
+  if (n == 0) return PS_OK;
+  long long g = (n + kIngThreads - 1) / kIngThreads;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kIngThreads), 0, st, log_unary_dev,
+                     extra_dev, n, weight);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(PS_ERR_HIP, std::string("extra unary: ") + hipGetErrorString(e));
   return PS_OK;
 }

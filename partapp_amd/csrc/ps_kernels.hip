@@ -23,6 +23,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "ps_device.hpp"
 #include "ps_rowgather.hpp"
@@ -659,13 +662,38 @@ size_t rowpass_lds_bytes(int R, int W, int kx) {
   return sizeof(float) * p;
 }
 
-hipError_t rowpass_set_lds_limit(size_t bytes) {
+// The dynamic-LDS attribute of a kernel is one process-wide value per device
+// and function, while plans with different needs coexist: it is only ever
+// raised (to the largest size asked for so far), under a lock, so a later
+// plan never lowers what an earlier plan's launches rely on.
+hipError_t raise_lds_limit(const void *f, size_t bytes) {
   if (bytes <= 64 * 1024) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rowpass_r<24>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, size_t> high;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_rowpass_any),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  std::lock_guard<std::mutex> lock(mu);
+  size_t &h = high[std::make_pair(dev, f)];
+  if (bytes <= h) return hipSuccess;
+  e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) h = bytes;
+  return e;
+}
+
+hipError_t rowpass_set_lds_limit(size_t bytes) {
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<24>), bytes);
+  if (e != hipSuccess) return e;
+  return raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_any), bytes);
+}
+
+hipError_t colpass_set_lds_limit(size_t bytes) {
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpRoot>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNode>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNodeU>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownLeaf>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownMid>), bytes);
+  return e;
 }
 
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,

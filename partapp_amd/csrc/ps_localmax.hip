@@ -218,7 +218,9 @@ extern "C" int ps_find_local_max(const float *grids_dev, const long long *offset
   int *cnt = reinterpret_cast<int *>(base + o_cnt);
   unsigned long long *cand = reinterpret_cast<unsigned long long *>(base + o_cand);
   hipError_t e = hipMemcpyAsync(gdev, gd.data(), sizeof(LmGrid) * n_grids, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) {  // counters cleared by a kernel (no memset node if a caller captures the stream)
+  // (this entry point allocates its workspace, uploads the grid table from
+  // pageable memory and synchronizes: it is synchronous and not capturable)
+  if (e == hipSuccess) {  // counters cleared by a small kernel
     hipLaunchKernelGGL(k_lmax_clear, dim3((n_grids + 255) / 256), dim3(256), 0, s, cnt, n_grids);
     e = hipGetLastError();
   }

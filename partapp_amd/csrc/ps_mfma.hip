@@ -790,11 +790,10 @@ size_t colpass_f16_lds_bytes(int KK) {
   return 2 * 8 * 2 * (size_t)f16_tap_copy_len(KK) + 2 * (size_t)64 * mfma_ring_pitch(KK);
 }
 
+hipError_t raise_lds_limit(const void *f, size_t bytes);
+
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
-  auto set = [](const void *f, size_t bytes) {
-    if (bytes <= 64 * 1024) return hipSuccess;
-    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  };
+  auto set = [](const void *f, size_t bytes) { return raise_lds_limit(f, bytes); };
   hipError_t e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);

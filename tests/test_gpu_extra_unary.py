@@ -1,0 +1,125 @@
+"""The extra unaries on the device (ps_add_torso_pos_prior, ps_add_dpm_score,
+ps_add_extra_unary) against the oracle's restatements of setTorsoPosPrior,
+addLoadDPMScore / addDPMScore and addExtraUnary (objectdetect_icps.cpp:
+137-191, 445-548) on the same inputs, then a sweep on unaries built with all
+of them against the oracle at the FP32 bar.
+
+Tolerances: the weighted sums are exact (the same float operations); where
+the reference takes exp (double) and log (float) the device's libm may
+differ from the host's by an ulp, so those values are compared to 2 float
+ulps."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from partapp_amd.api import (DPM_LOG, DPM_LOG_OF_RAW, DPM_RAW, ExpParam, add_dpm_score, add_extra_unary,
+                             person_model, set_torso_pos_prior, synth_joints, synth_unary)
+
+from test_gpu_parity import check_parity, run_plan
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ulps(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+def test_torso_prior_matches_oracle(gpu):
+    import torch
+    rng = np.random.default_rng(21)
+    g = rng.normal(-3, 0.5, (3, 24, 61, 77)).astype(np.float32)
+    params = (7.5, -12.25, 120.0, 310.0)
+    say("torso prior 3x24x61x77", params)
+    d = torch.from_numpy(g).to(gpu)
+    set_torso_pos_prior(d, params, 0.6)
+    got = d.cpu().numpy()
+    ref = oracle.torso_pos_prior(g, params, 0.6)
+    u = ulps(got, ref)
+    say("  max ulps %d, exact %.4f" % (u.max(), np.mean(u == 0)))
+    assert u.max() <= 2
+
+
+@pytest.mark.parametrize("n_dpm", [1, 24, 5])
+def test_dpm_raw_matches_oracle(gpu, n_dpm):
+    import torch
+    rng = np.random.default_rng(22 + n_dpm)
+    g = rng.normal(-3, 0.5, (2, 24, 33, 45)).astype(np.float32)
+    dpm = rng.uniform(-1e-4, 3e-4, (n_dpm, 33, 45)).astype(np.float32)  # both sides of the 1e-4 clip
+    dpm[0, 0, :5] = [0.0, 1e-4, 1e-5, -1.0, 1e-4 + 1e-9]
+    say("dpm raw, %d slices" % n_dpm)
+    d = torch.from_numpy(g).to(gpu)
+    add_dpm_score(d, torch.from_numpy(dpm).to(gpu), 1.7, DPM_RAW)
+    ref = oracle.add_dpm_raw(g, dpm, 1.7)
+    assert ulps(d.cpu().numpy(), ref).max() <= 2
+
+
+@pytest.mark.parametrize("n_dpm", [1, 24])
+def test_dpm_log_matches_oracle(gpu, n_dpm):
+    import torch
+    rng = np.random.default_rng(40 + n_dpm)
+    g = rng.normal(-3, 0.5, (3, 24, 20, 28)).astype(np.float32)
+    raw = rng.uniform(0.0, 2.0, (n_dpm, 20, 28)).astype(np.float32)
+    raw[0, 3, :4] = 0.0
+    say("dpm log, %d slices" % n_dpm)
+    # addDPMScore on log grids: the same float operations, bit for bit
+    lg = oracle.log_grid(raw)
+    d = torch.from_numpy(g).to(gpu)
+    add_dpm_score(d, torch.from_numpy(lg).to(gpu), 0.9, DPM_LOG)
+    ref = oracle.add_dpm_log(g, lg, 0.9)
+    assert np.array_equal(d.cpu().numpy(), ref)
+    # computeLogGrid of raw scores fused in (the log may differ by an ulp)
+    d2 = torch.from_numpy(g).to(gpu)
+    add_dpm_score(d2, torch.from_numpy(raw).to(gpu), 0.9, DPM_LOG_OF_RAW)
+    assert ulps(d2.cpu().numpy(), ref).max() <= 2
+
+
+def test_extra_unary_is_exact(gpu):
+    import torch
+    rng = np.random.default_rng(23)
+    g = rng.normal(-3, 0.5, (10, 1, 24, 17, 19)).astype(np.float32)
+    e = rng.normal(0, 2.0, g.shape).astype(np.float32)
+    d = torch.from_numpy(g).to(gpu)
+    add_extra_unary(d, torch.from_numpy(e).to(gpu), 0.35)
+    assert np.array_equal(d.cpu().numpy(), oracle.add_extra_unary(g, e, 0.35))
+
+
+def test_sweep_with_extra_unaries(gpu):
+    # findObjectImageRotJoints' order (objectdetect_findrot.cpp:896-948): DPM
+    # torso score (log of raw), DPM head score (raw), an extra unary over all
+    # parts, the torso position prior; then the sweep
+    import torch
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(24)
+    model = person_model(ep, joints)
+    H, W = 112, 128
+    u = synth_unary(24, 10, 1, 24, H, W)
+    rng = np.random.default_rng(25)
+    torso_dpm = rng.uniform(0.0, 1.0, (1, H, W)).astype(np.float32)
+    head_dpm = rng.uniform(0.0, 1e-3, (1, H, W)).astype(np.float32)
+    extra = rng.normal(0.0, 1.0, u.shape).astype(np.float32)
+    root, head = model.root, 5
+    params = (3.0, -8.0, 400.0, 900.0)
+    d = torch.from_numpy(u).to(gpu)
+    add_dpm_score(d[root], torch.from_numpy(torso_dpm).to(gpu), 0.5, DPM_LOG_OF_RAW)
+    add_dpm_score(d[head], torch.from_numpy(head_dpm).to(gpu), 0.3, DPM_RAW)
+    add_extra_unary(d, torch.from_numpy(extra).to(gpu), 0.2)
+    set_torso_pos_prior(d[root], params, 0.9)
+    v = u.copy()
+    v[root] = oracle.add_dpm_log(v[root], oracle.log_grid(torso_dpm), 0.5)
+    v[head] = oracle.add_dpm_raw(v[head], head_dpm, 0.3)
+    v = oracle.add_extra_unary(v, extra, 0.2)
+    v[root] = oracle.torso_pos_prior(v[root], params, 0.9)
+    dv = d.cpu().numpy()
+    assert ulps(dv, v).max() <= 4
+    say("sweep on unaries with extra terms")
+    # the sweep reads the device's unaries; the oracle the same values
+    out = run_plan(model, [dv])[0]
+    check_parity("extra unaries", model, dv, joints, out)

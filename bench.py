@@ -60,6 +60,7 @@ def parse():
                     help="instances checked against the oracles (default: all of config 1, 16 of config 2)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-guard", action="store_true", help="skip the MAP-guarded run of the chosen method")
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel roofline)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 256 MiB (the known-size kernel that scales the rocprofv3 "
@@ -216,15 +217,18 @@ def cpu_baseline_timed(ep, sizes, joints, seeds, budget_s):
     return images, busy, cores
 
 
-def time_method(plan, method, unary, marg, root, maps, stream, args, steps=None, warmup=None, classes=()):
+def time_method(plan, method, unary, marg, root, maps, stream, args, steps=None, warmup=None, classes=(),
+                guard=0.0):
     """Warmup, then EXACTLY `steps` (default args.steps) timed steps, the
     launches of the kernel classes in `classes` bracketed by HIP events on the
-    stream they run on; returns (ms, profile or None)."""
+    stream they run on; `guard`: the MAP guard's threshold (0: off); returns
+    (ms, profile or None)."""
     import torch
     steps = args.steps if steps is None else steps
     warmup = args.warmup if warmup is None else warmup
     profile = bool(classes) and not args.graph
     plan.set_method(method)
+    plan.set_map_guard(guard)
     with torch.cuda.stream(stream):
         for _ in range(warmup):
             plan.execute(unary, marg, root, maps, stream=stream, graph=args.graph)
@@ -418,6 +422,22 @@ def main():
     value = n * args.steps / (ms / 1e3)
     print("%s: timed %d steps, %.3f ms/step" % (args.method, args.steps, ms_per_step), file=sys.stderr, flush=True)
 
+    # the same method with the MAP guard: instances whose MAP certificate
+    # fails are swept again in FP32 within the step
+    guarded = None
+    tau = Plan.MAP_GUARD_TAU.get(args.method, 0.0)
+    if tau > 0 and not args.no_guard:
+        margg = torch.empty_like(unary)
+        mapsg = torch.zeros_like(maps)
+        msg_, _ = time_method(plan, args.method, unary, margg, root, mapsg, stream, args, guard=tau)
+        n_re, _ = plan.read_map_guard(stream)
+        plan.set_map_guard(0.0)
+        guarded = {"tau": tau, "value": round(n * args.steps / (msg_ / 1e3), 3),
+                   "ms_per_step": round(msg_ / args.steps, 4), "reevaluated_in_fp32": "%d/%d" % (n_re, n)}
+        legs[args.method + "+guard"] = (margg, mapsg)
+        print("%s with the MAP guard (tau %g): %.3f ms/step, %d of %d instances re-evaluated" % (
+            args.method, tau, msg_ / args.steps, n_re, n), file=sys.stderr, flush=True)
+
     err = None
     k = n if args.check is None else min(args.check, n)
     if k > 0:
@@ -427,6 +447,8 @@ def main():
         err = chk[args.method]
         if fp32 is not None:
             fp32["parity"] = chk["fp32"]
+        if guarded is not None:
+            guarded["error_vs_fp64"] = chk[args.method + "+guard"]
         print("checked %d instances in %.1f s: %s" % (k, time.perf_counter() - tc, json.dumps(chk)), file=sys.stderr,
               flush=True)
 
@@ -518,6 +540,7 @@ def main():
                    "autotune": tuned},
         "plain_fp32": fp32,
         "vs_plain_fp32": round(value / fp32["value"], 3) if fp32 else None,
+        "map_guard": guarded,
         "error_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -163,6 +163,19 @@ enum { PS_TUNE_MFMA_ROWS = 1 };
 int ps_plan_tune(ps_plan *plan, int knob, int value);
 int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
 
+/* MAP guard of the BF16 methods (no counterpart in the reference, whose MAP
+ * is the FP32 argmax).  With tau > 0 and a BF16 method, every execution also
+ * keeps, per (instance, scale, part), the best and second best log-marginal
+ * values; an instance where some best does not beat its second best by more
+ * than tau (twice the method's bound on the error of a log-marginal value)
+ * is swept again with the FP32 passes, on the device, before the execution
+ * ends (its marginals, root posterior and MAP records then come from FP32).
+ * tau = 0 turns the guard off (the default).  ps_plan_read_map_guard
+ * reports how many instances the last execution re-evaluated (and, if
+ * flags_host is given, which: [n_instances]); it waits for `stream`. */
+int ps_plan_set_map_guard(ps_plan *plan, float tau);
+int ps_plan_read_map_guard(ps_plan *plan, int *n_reevaluated, unsigned *flags_host, void *stream);
+
 /* Per-kernel timing of a plan's executions, for measurement: while enabled,
  * every kernel launch of ps_plan_execute (a level's eigenbasis stages count as
  * one) is bracketed by HIP events on its stream.

@@ -108,6 +108,9 @@ SIGNATURES = {
                                              ctypes.c_void_p, ctypes.c_void_p]),
     "ps_plan_set_method": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_plan_method_levels": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
+    "ps_plan_set_map_guard": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    "ps_plan_read_map_guard": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.POINTER(ctypes.c_uint),
+                                              ctypes.c_void_p]),
     "ps_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "ps_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_plan_set_profiling_classes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),

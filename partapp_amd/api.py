@@ -224,6 +224,25 @@ class Plan:
         check(lib().ps_plan_method_levels(self.handle, ctypes.byref(a), ctypes.byref(b)), "ps_plan_method_levels")
         return a.value, b.value
 
+    # certificate thresholds of the MAP guard per method: above twice the
+    # largest spread of the log-marginal error (relative to the MAP state's)
+    # measured against the FP64 oracle near each part's maximum
+    MAP_GUARD_TAU = {"fp32": 0.0, "bf16": 4e-3, "bf16mix": 4e-3, "bf16x2": 4e-3, "bf16x3": 1e-4}
+
+    def set_map_guard(self, tau):
+        """MAP guard of the BF16 methods: instances whose best log marginal
+        of some part beats the second best by at most tau are swept again in
+        FP32 on the device (0 turns it off)."""
+        check(lib().ps_plan_set_map_guard(self.handle, float(tau)), "ps_plan_set_map_guard")
+
+    def read_map_guard(self, stream=None):
+        """(instances re-evaluated by the last execution, per-instance flags)."""
+        n = ctypes.c_int()
+        flags = (ctypes.c_uint * self.n_instances)()
+        check(lib().ps_plan_read_map_guard(self.handle, ctypes.byref(n), flags, _stream_handle(stream)),
+              "ps_plan_read_map_guard")
+        return n.value, [int(f) for f in flags]
+
     TUNE_MFMA_ROWS = 1
 
     def tune(self, knob, value):

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -21,7 +22,10 @@
 
 namespace ps {
 extern thread_local std::string g_last_error;
-hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map, hipStream_t s);
+hipError_t launch_reset(const Bases &b, int n_keys, int n_map, int n_inst, unsigned *flags, int n_flags,
+                        hipStream_t s);
+hipError_t launch_certify(const unsigned long long *top2, int n_map, int n_inst, float tau, unsigned *flags,
+                          hipStream_t s);
 hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
 size_t rowpass_lds_bytes(int R, int W, int kx);
 hipError_t rowpass_set_lds_limit(size_t bytes);
@@ -146,6 +150,11 @@ struct ps_plan {
   int n_keys = 0;
   unsigned long long *map_keys = nullptr;
   int n_map = 0;
+  // MAP guard of the BF16 methods: top-2 value keys per argmax key, a flag
+  // per instance, and the certificate threshold (0: off)
+  unsigned long long *top2 = nullptr;
+  unsigned *flags = nullptr;
+  float guard_tau = 0.0f;
   std::vector<Level> up, down;
   int n_roots = 0, n_maxes = 0;
   int root_children = 0;  // the same for every root job
@@ -454,8 +463,10 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       for (int k = 0; k < K; ++k) {
         int leaf = chains[k].back();
         MaxJob mj;
+        std::memset(&mj, 0, sizeof(mj));
         mj.H = H;
         mj.W = W;
+        mj.inst = i;
         mj.src_kind = m.is_detect[leaf] ? kSrcUnary : kSrcZero;
         mj.key = up_key(leaf);
         mj.src = unary(leaf);
@@ -509,6 +520,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           std::memset(&mj, 0, sizeof(mj));
           mj.H = H;
           mj.W = W;
+          mj.inst = i;
           B.add_message(mp, mj);
           mj.od_buf = mp.od ? odbuf(k) : kNone;
           if (t.height[c] == 1) {
@@ -543,6 +555,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         std::memset(&rj, 0, sizeof(rj));
         rj.H = H;
         rj.W = W;
+        rj.inst = i;
         rj.K = K;
         rj.add_unary = m.is_detect[t.root];
         for (int k = 0; k < K; ++k) {
@@ -576,6 +589,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           std::memset(&mj, 0, sizeof(mj));
           mj.H = H;
           mj.W = W;
+          mj.inst = i;
           B.add_message(mp, mj);
           mj.od_buf = mp.od ? odbuf(k) : kNone;
           mj.src_kind = kSrcGrid;
@@ -600,8 +614,10 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       }
       for (int p = 0; p < P; ++p) {
         MapJob mj;
+        std::memset(&mj, 0, sizeof(mj));
         mj.H = H;
         mj.W = W;
+        mj.inst = i;
         mj.slot = slot(p);
         mj.out = it * P + p;
         B.maps.push_back(mj);
@@ -710,7 +726,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   size_t o_frags = align(o_shifts + B.shifts.size() * sizeof(int));
   size_t o_keys = align(o_frags + B.frags.size() * sizeof(uint16_t));
   size_t o_mkeys = align(o_keys + (size_t)pl->n_keys * sizeof(unsigned));
-  size_t total = align(o_mkeys + (size_t)pl->n_map * sizeof(unsigned long long));
+  size_t o_top2 = align(o_mkeys + (size_t)pl->n_map * sizeof(unsigned long long));
+  size_t o_flags = align(o_top2 + (size_t)pl->n_map * sizeof(unsigned long long));
+  size_t total = align(o_flags + (size_t)n_instances * sizeof(unsigned));
   hipError_t e = hipMalloc(&pl->tables, total);
   if (e != hipSuccess) {
     delete pl;
@@ -726,6 +744,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->frags = reinterpret_cast<uint4 *>(base + o_frags);
   pl->keys = reinterpret_cast<unsigned *>(base + o_keys);
   pl->map_keys = reinterpret_cast<unsigned long long *>(base + o_mkeys);
+  pl->top2 = reinterpret_cast<unsigned long long *>(base + o_top2);
+  pl->flags = reinterpret_cast<unsigned *>(base + o_flags);
   std::vector<char> host(total, 0);
   std::memcpy(host.data() + o_jobs, B.jobs.data(), B.jobs.size() * sizeof(MsgJob));
   std::memcpy(host.data() + o_roots, B.roots.data(), B.roots.size() * sizeof(RootJob));
@@ -789,6 +809,33 @@ extern "C" int ps_plan_set_method(ps_plan *plan, int method) {
   }
   plan->terms = terms;
   plan->col_terms = col_terms;
+  return PS_OK;
+}
+
+extern "C" int ps_plan_set_map_guard(ps_plan *plan, float tau) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  if (!(tau >= 0.0f) || !std::isfinite(tau)) return set_error(PS_ERR_INVALID_ARG, "tau must be finite and >= 0");
+  if (tau != plan->guard_tau && plan->gexec) {
+    (void)hipGraphExecDestroy(plan->gexec);
+    plan->gexec = nullptr;
+  }
+  plan->guard_tau = tau;
+  return PS_OK;
+}
+
+extern "C" int ps_plan_read_map_guard(ps_plan *plan, int *n_reevaluated, unsigned *flags_host, void *stream) {
+  if (!plan || !n_reevaluated) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  std::vector<unsigned> f(plan->n_inst);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpyAsync(f.data(), plan->flags, sizeof(unsigned) * f.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_error(e, "reading the MAP guard flags");
+  int n = 0;
+  for (int i = 0; i < plan->n_inst; ++i) {
+    n += f[i] ? 1 : 0;
+    if (flags_host) flags_host[i] = f[i];
+  }
+  *n_reevaluated = n;
   return PS_OK;
 }
 
@@ -861,18 +908,12 @@ extern "C" size_t ps_plan_scratch_bytes(const ps_plan *plan) {
   return plan ? plan->scratch_elems * sizeof(float) : 0;
 }
 
-static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, float *root_posterior, ps_map *map_dev,
-                   hipStream_t s, bool profile) {
-  Bases b;
-  b.unary = log_unary;
-  b.marginal = log_marginal;
-  b.scratch = pl->scratch;
-  b.root = root_posterior;
-  b.max_keys = pl->keys;
-  b.map_keys = pl->map_keys;
-  ps_plan *pp = profile ? pl : nullptr;
+// One sweep of the plan with the spatial passes of `terms` / `col_terms`
+// (0: FP32) over the instances b.active names (all when null).
+static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, ps_map *map_dev, hipStream_t s,
+                        ps_plan *pp, unsigned *clear_flags) {
   long ev = profile_begin(pp, 0, s);
-  hipError_t e = launch_reset(pl->keys, pl->n_keys, pl->map_keys, pl->n_map, s);
+  hipError_t e = launch_reset(b, pl->n_keys, pl->n_map, pl->n_inst, clear_flags, pl->n_inst, s);
   profile_end(pp, ev, s);
   if (e == hipSuccess) {
     ev = profile_begin(pp, 1, s);
@@ -880,9 +921,9 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     profile_end(pp, ev, s);
   }
   auto level = [&](const Level &L) {
-    const bool mf = pl->terms != 0 && L.mfma_ok;
-    const int ti = pl->terms > 0 ? pl->terms - 1 : 0;
-    const int tc = pl->col_terms > 0 ? pl->col_terms - 1 : 0;
+    const bool mf = terms != 0 && L.mfma_ok;
+    const int ti = terms > 0 ? terms - 1 : 0;
+    const int tc = col_terms > 0 ? col_terms - 1 : 0;
     const MsgJob *jobs = pl->jobs + L.off;
     const int n_od = L.count - L.n_diag;
     // every kernel launch (the eigenbasis stages of a level as one) has its
@@ -891,7 +932,7 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
       if (e != hipSuccess) break;
       const long ev2 = profile_begin(pp, 2, s);
       if (mf)
-        e = launch_rowpass_mfma(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->frags, pl->R, pl->terms,
+        e = launch_rowpass_mfma(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->frags, pl->R, terms,
                                 pl->mfma_rows, c.max_h, c.max_w, L.mfma_row_lds[ti], s);
       else
         e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s);
@@ -906,14 +947,13 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     // the fp16 and FP32 column passes have one instantiation per highest
     // epilogue kind: they launch per (width, kind) class; the bf16 three-term
     // column pass per width
-    const bool per_kind = !mf || pl->terms != 3;
+    const bool per_kind = !mf || terms != 3;
     for (const WidthClass &c : per_kind ? L.col_classes : L.classes) {
       if (e != hipSuccess) break;
       const long ev2 = profile_begin(pp, 3, s);
       if (mf)
-        e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, pl->col_terms,
-                                pl->terms != 3, c.epi_hi, c.max_w,
-                                pl->terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
+        e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, col_terms, terms != 3,
+                                c.epi_hi, c.max_w, terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
       else
         e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s);
       profile_end(pp, ev2, s);
@@ -942,6 +982,36 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
     ev = profile_begin(pp, 5, s);
     e = launch_map_decode(b, pl->maps, pl->n_map, map_dev, s);
     profile_end(pp, ev, s);
+  }
+  return e;
+}
+
+// The plan's method; with the MAP guard on and a BF16 method, every
+// instance whose MAP certificate fails (some part's best log marginal beats
+// its second best by at most guard_tau) is swept again in FP32, on the
+// device, in the same stream order.
+static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, float *root_posterior, ps_map *map_dev,
+                   hipStream_t s, bool profile) {
+  const bool guard = pl->guard_tau > 0.0f && pl->terms != 0;
+  Bases b;
+  b.unary = log_unary;
+  b.marginal = log_marginal;
+  b.scratch = pl->scratch;
+  b.root = root_posterior;
+  b.max_keys = pl->keys;
+  b.map_keys = pl->map_keys;
+  b.top2_keys = guard ? pl->top2 : nullptr;
+  b.active = nullptr;
+  ps_plan *pp = profile ? pl : nullptr;
+  hipError_t e = sweep(pl, b, pl->terms, pl->col_terms, map_dev, s, pp, pl->flags);
+  if (e == hipSuccess && guard) {
+    e = launch_certify(pl->top2, pl->n_map, pl->n_inst, pl->guard_tau, pl->flags, s);
+    if (e == hipSuccess) {
+      Bases bf = b;
+      bf.top2_keys = nullptr;
+      bf.active = pl->flags;
+      e = sweep(pl, bf, 0, 0, map_dev, s, pp, nullptr);
+    }
   }
   if (e != hipSuccess) return hip_error(e, "kernel launch");
   return PS_OK;
@@ -1114,12 +1184,14 @@ int message_enqueue(const MessageSetup &ms, char *mem, const float *child_dev, f
   b.root = nullptr;
   b.max_keys = reinterpret_cast<unsigned *>(mem + ms.o_key);
   b.map_keys = reinterpret_cast<unsigned long long *>(mem + ms.o_mk);
+  b.top2_keys = nullptr;
+  b.active = nullptr;
   const MsgJob *jd = reinterpret_cast<const MsgJob *>(mem + ms.o_job);
   const float *td = reinterpret_cast<const float *>(mem + ms.o_taps);
   const int *sd = reinterpret_cast<const int *>(mem + ms.o_sh);
   if (e == hipSuccess) e = rowpass_set_lds_limit(ms.lds);
   if (e == hipSuccess && !ms.od) e = colpass_set_lds_limit(ms.col_lds);
-  if (e == hipSuccess) e = launch_reset(b.max_keys, 1, b.map_keys, 1, s);
+  if (e == hipSuccess) e = launch_reset(b, 1, 1, 1, nullptr, 0, s);
   if (e == hipSuccess)
     e = launch_max(b, reinterpret_cast<const MaxJob *>(mem + ms.o_max), 1, ms.R, ms.H * ms.W, s);
   if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, ms.R, ms.H, ms.W, ms.lds, s);

@@ -54,6 +54,43 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return v;
 }
 
+// An instance left out of this execution (Bases::active names the instances
+// that run): every block of its jobs exits at once.
+__device__ __forceinline__ bool skip_inst(const Bases &b, int inst) { return b.active && b.active[inst] == 0u; }
+
+// Top-2 value keys of a wave's outputs (per lane k1 >= k2, every state seen
+// by one lane only), merged across the wave and then into a slot's
+// (best << 32) | second word with a compare-and-swap loop; the word only
+// grows, so the loop ends.  Every lane of the wave calls it.
+__device__ __forceinline__ void top2_flush(unsigned long long *slot, unsigned k1, unsigned k2, int lane) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned o1 = (unsigned)__shfl_xor((int)k1, o, 64), o2 = (unsigned)__shfl_xor((int)k2, o, 64);
+    k2 = max(max(k2, o2), min(k1, o1));
+    k1 = max(k1, o1);
+  }
+  if (lane == 0 && k1) {
+    unsigned long long old = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+      const unsigned a1 = (unsigned)(old >> 32), a2 = (unsigned)old;
+      const unsigned long long nw = ((unsigned long long)max(a1, k1) << 32) | max(max(a2, k2), min(a1, k1));
+      if (nw == old) break;
+      const unsigned long long prev = atomicCAS(slot, old, nw);
+      if (prev == old) break;
+      old = prev;
+    }
+  }
+}
+// one more value key into a lane's top-2
+__device__ __forceinline__ void top2_add(unsigned &k1, unsigned &k2, unsigned k) {
+  if (k > k1) {
+    k2 = k1;
+    k1 = k;
+  } else if (k > k2) {
+    k2 = k;
+  }
+}
+
 // Base pointer of a GridRef.  Written as selects on the (uniform) buffer id
 // so the kernel argument block is never copied to scratch for indexing.
 __device__ __forceinline__ float *base_of(const Bases &b, int buf) {
@@ -82,6 +119,7 @@ struct Epi {
   static constexpr bool kDown = EPI >= kEpiDownLeaf;
   float vmax = -INFINITY;  // max of the written next source / belief
   unsigned best_key = 0u, best_eb = 0u;  // argmax of the marginal (key, byte offset in the slice)
+  unsigned second_key = 0u;              // second largest marginal value (key) of the lane
 
   // one output at byte offset eb of slice r (dst, dst2 point at the slice);
   // pu / pb are only read when the kind needs them
@@ -101,8 +139,11 @@ struct Epi {
       at(dst, eb) = mg;
       const unsigned k = f2key(mg);
       if (k > best_key) {  // a lane's outputs come in ascending flat order: the first maximum stays
+        second_key = best_key;
         best_key = k;
         best_eb = eb;
+      } else if (k > second_key) {
+        second_key = k;
       }
       if (kNeedB) {
         const float v2 = u + msg;
@@ -122,6 +163,7 @@ struct Epi {
       const unsigned long long m =
           wave_max_u64(best_key ? ((unsigned long long)best_key << 32) | (0xffffffffull - flat) : 0ull);
       if (lane == 0 && m) atomicMax(b.map_keys + J.map_slot, m);
+      if (b.top2_keys) top2_flush(b.top2_keys + J.map_slot, best_key, second_key, lane);
     }
   }
 };
@@ -131,6 +173,7 @@ struct Epi {
 struct DstAcc {
   unsigned kmax1 = 0u, kmax2 = 0u;
   unsigned long long amax = 0ull;
+  unsigned second = 0u;  // second largest marginal value (key)
 };
 
 // The caller's accumulation of one message value msg at element
@@ -157,6 +200,8 @@ __device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *ds
     const float mg = base + msg;
     dst[e] = mg;
     const unsigned long long k = amax_key(mg, flat);
+    const unsigned kv = (unsigned)(k >> 32), kb = (unsigned)(a.amax >> 32);
+    a.second = kv > kb ? kb : max(a.second, kv);
     a.amax = k > a.amax ? k : a.amax;
     if (dst2) {
       const float v2 = u + msg;
@@ -175,6 +220,7 @@ __device__ __forceinline__ void dst_flush(const Bases &b, const MsgJob &J, bool 
   } else if (J.dst_kind == kDstDown) {
     const unsigned long long m = wave_max_u64(a.amax);
     if (lane == 0 && m) atomicMax(b.map_keys + J.map_slot, m);
+    if (b.top2_keys) top2_flush(b.top2_keys + J.map_slot, (unsigned)(a.amax >> 32), a.second, lane);
     if (has_dst2) {
       const unsigned v = wave_max_u32(a.kmax2);
       if (lane == 0 && v) atomicMax(b.max_keys + J.dst2_key, v);

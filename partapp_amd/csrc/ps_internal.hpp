@@ -31,6 +31,8 @@ struct Bases {
   float *root;
   unsigned *max_keys;            // order-preserving keys of grid maxima
   unsigned long long *map_keys;  // (score, first index) argmax keys
+  unsigned long long *top2_keys; // per argmax key: (best value key << 32) | second best value key, or null
+  const unsigned *active;        // per instance: 0 = skip it in this execution; null = run every instance
 };
 
 // Upright / border masking of one part's unaries
@@ -50,6 +52,8 @@ enum RotMode : int { kRotFilter = 0, kRotCopy = 1, kRotZero = 2 };
 // accumulation its caller performs on the result.
 struct MsgJob {
   int H, W;
+  int inst;      // instance of the plan
+  int pad0_;
   int delta;     // rotation shift: output slice = source slice + delta
   int rot_mode;  // RotMode
   int rot_len;   // taps of the circular rotation filter
@@ -137,6 +141,8 @@ struct RootJob {
   int H, W;
   int K;  // root children, in joint order
   int add_unary;
+  int inst;
+  int pad0_;
   GridRef msg[kMaxRootChildren];  // upward messages; overwritten with log_from_root
   int fr_key[kMaxRootChildren];
   GridRef unary;
@@ -153,6 +159,8 @@ struct MaxJob {
   int H, W;
   int src_kind;
   int key;
+  int inst;
+  int pad0_;
   GridRef src;
   UnaryMask mask;
 };
@@ -162,6 +170,8 @@ struct MapJob {
   int H, W;
   int slot;
   int out;  // index into the ps_map array
+  int inst;
+  int pad0_;
 };
 
 }  // namespace ps

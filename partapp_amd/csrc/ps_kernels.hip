@@ -36,10 +36,19 @@ namespace ps {
 
 // --------------------------------------------------------------- k_reset
 
-__global__ void k_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map) {
+// Clears the max keys, the argmax keys and their top-2 keys of the instances
+// that run (keys_per_inst / map_per_inst consecutive entries per instance);
+// with clear_active it also clears the per-instance flags (n_flags) that a
+// later certification pass sets.
+__global__ void k_reset(Bases b, int n_keys, int n_map, int keys_per_inst, int map_per_inst, unsigned *flags,
+                        int n_flags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_keys) keys[i] = 0u;
-  if (i < n_map) map_keys[i] = 0ull;
+  if (i < n_keys && !skip_inst(b, i / keys_per_inst)) b.max_keys[i] = 0u;
+  if (i < n_map && !skip_inst(b, i / map_per_inst)) {
+    b.map_keys[i] = 0ull;
+    if (b.top2_keys) b.top2_keys[i] = 0ull;
+  }
+  if (flags && i < n_flags) flags[i] = 0u;
 }
 
 // ------------------------------------------------------------------ k_max
@@ -48,6 +57,7 @@ __global__ void k_reset(unsigned *keys, int n_keys, unsigned long long *map_keys
 
 __global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__ jobs, int R, int chunk) {
   const MaxJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   const int H = J.H, W = J.W;
   const long long n = (long long)R * H * W;
   const long long e0 = (long long)blockIdx.x * chunk;
@@ -177,6 +187,7 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
   unsigned bx, by;
   xcd_block(bx, by);
   const MsgJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int y = (int)bx;
   const int H = J.H, W = J.W;
   if (y >= H) return;
@@ -252,6 +263,7 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
   unsigned bx, by;
   xcd_block(bx, by);
   const MsgJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int y = (int)bx;
   const int H = J.H, W = J.W;
   if (y >= H) return;
@@ -492,6 +504,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   unsigned bx, by;
   xcd_block(bx, by);
   const MsgJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int r = (int)bx / xtiles;
   const int xt = (int)bx - r * xtiles;
   if (xt * kColTileX >= J.W) return;
@@ -528,6 +541,7 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
 #pragma unroll
   for (int k = 0; k < KC; ++k) kmax[k] = 0u;
   unsigned long long amax = 0ull;
+  unsigned second = 0u;  // second largest root belief (key), for the MAP certificate
   float sum = 0.0f;
   bool first = true;
 
@@ -555,6 +569,8 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
     if (ok) {
       marg[e] = bel;
       const unsigned long long ak = amax_key(bel, (unsigned)e);
+      const unsigned kv = (unsigned)(ak >> 32), kb = (unsigned)(amax >> 32);
+      second = kv > kb ? kb : max(second, kv);
       amax = ak > amax ? ak : amax;
     }
     // log_from_root of child i: the other messages in order (:641-649), then
@@ -586,6 +602,7 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
   if (ok && J.root_post.buf >= 0 && b.root) mptr(b, J.root_post)[pix] = log0(sum);
 
   const int lane = threadIdx.x & 63;
+  if (b.top2_keys) top2_flush(b.top2_keys + J.map_slot, (unsigned)(amax >> 32), second, lane);
   amax = wave_max_u64(amax);
   if (lane == 0 && amax) atomicMax(b.map_keys + J.map_slot, amax);
 #pragma unroll
@@ -605,6 +622,7 @@ __global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict
   unsigned bx, by;
   xcd_block(bx, by);
   const RootJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int pix0 = (int)bx * blockDim.x;
   if (pix0 >= J.H * J.W) return;
   root_pixels<KC>(b, J, R, pix0);
@@ -616,6 +634,7 @@ __global__ void k_map_decode(Bases b, const MapJob *__restrict__ jobs, int n, ps
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const MapJob J = jobs[i];
+  if (skip_inst(b, J.inst)) return;
   const unsigned long long k = b.map_keys[J.slot];
   ps_map m;
   if (k == 0ull) {
@@ -635,12 +654,39 @@ __global__ void k_map_decode(Bases b, const MapJob *__restrict__ jobs, int n, ps
   out[J.out] = m;
 }
 
+// ------------------------------------------------------------- k_certify
+// MAP certificate of one (instance, scale, part): with every log-marginal
+// value off by at most eps, the computed argmax is the exact one whenever
+// the computed best beats the second best by more than tau = 2 eps.  An
+// instance with any slot at or below tau is flagged for re-evaluation.
+
+__global__ void k_certify(const unsigned long long *__restrict__ top2, int n_map, int map_per_inst, float tau,
+                          unsigned *flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_map) return;
+  const unsigned long long t = top2[i];
+  const unsigned k1 = (unsigned)(t >> 32), k2 = (unsigned)t;
+  const float v1 = k1 ? key2f(k1) : -INFINITY;
+  const float v2 = k2 ? key2f(k2) : -INFINITY;
+  if (!(v1 - v2 > tau)) flags[i / map_per_inst] = 1u;
+}
+
 // ------------------------------------------------------------ host launchers
 
-hipError_t launch_reset(unsigned *keys, int n_keys, unsigned long long *map_keys, int n_map, hipStream_t s) {
-  int n = n_keys > n_map ? n_keys : n_map;
+hipError_t launch_certify(const unsigned long long *top2, int n_map, int n_inst, float tau, unsigned *flags,
+                          hipStream_t s) {
+  if (n_map <= 0) return hipSuccess;
+  const int mpi = std::max(1, n_map / std::max(1, n_inst));
+  hipLaunchKernelGGL(k_certify, dim3((n_map + 255) / 256), dim3(256), 0, s, top2, n_map, mpi, tau, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset(const Bases &b, int n_keys, int n_map, int n_inst, unsigned *flags, int n_flags,
+                        hipStream_t s) {
+  int n = std::max(std::max(n_keys, n_map), flags ? n_flags : 0);
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, keys, n_keys, map_keys, n_map);
+  const int kpi = std::max(1, n_keys / std::max(1, n_inst)), mpi = std::max(1, n_map / std::max(1, n_inst));
+  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, s, b, n_keys, n_map, kpi, mpi, flags, n_flags);
   return hipGetLastError();
 }
 

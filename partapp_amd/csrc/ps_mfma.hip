@@ -133,6 +133,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
   unsigned bx, by;
   xcd_block(bx, by);
   const MsgJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int H = J.H, W = J.W;
   const int y0 = (int)bx * rows;
   if (y0 >= H) return;
@@ -736,6 +737,7 @@ __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__re
   unsigned bx, by;
   xcd_block(bx, by);
   const MsgJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int r = (int)bx / xtiles;
   const int xt = (int)bx - r * xtiles;
   if (xt * 64 >= J.W) return;
@@ -760,6 +762,7 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
   unsigned bx, by;
   xcd_block(bx, by);
   const MsgJob &J = jobs[by];
+  if (skip_inst(b, J.inst)) return;
   const int r = (int)bx / xtiles;
   const int xt = (int)bx - r * xtiles;
   if (xt * 64 >= J.W) return;

@@ -67,6 +67,7 @@ __device__ __forceinline__ float bilinear_at(const float *g, int h, int w, int p
 // Zero the scatter keys of the direct jobs.
 __global__ __launch_bounds__(kOdThreads) void k_od_clear(Bases b, const MsgJob *__restrict__ jobs, int R) {
   const MsgJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   if (!J.od_direct) return;
   const long long n = (long long)R * J.th * J.tw;
   unsigned long long *k = od_keys(b, J);
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_clear(Bases b, const MsgJob *
 // comes last in the reference's x-outer, y-inner scan.
 __global__ __launch_bounds__(kOdThreads) void k_od_scatter(Bases b, const MsgJob *__restrict__ jobs, int R) {
   const MsgJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   if (!J.od_direct) return;
   const int H = J.H, W = J.W, th = J.th, tw = J.tw;
   const long long HW = (long long)H * W;
@@ -110,6 +112,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_scatter(Bases b, const MsgJob
 // TM_BILINEAR: eigenbasis cell (x3, y3) samples slice r at T13 (x3, y3).
 __global__ __launch_bounds__(kOdThreads) void k_od_gather(Bases b, const MsgJob *__restrict__ jobs, int R) {
   const MsgJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   if (J.od_direct) return;
   const int H = J.H, W = J.W, th = J.th, tw = J.tw;
   const long long cells = (long long)th * tw;
@@ -134,6 +137,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_gather(Bases b, const MsgJob 
 __global__ __launch_bounds__(kOdThreads) void k_od_xfilter(Bases b, const MsgJob *__restrict__ jobs,
                                                            const float *__restrict__ taps, int R) {
   const MsgJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   const int th = J.th, tw = J.tw;
   const long long n = (long long)R * th * tw;
   const int k = J.kxe, L = 2 * J.kxe + 1;
@@ -167,6 +171,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_xfilter(Bases b, const MsgJob
 __global__ __launch_bounds__(kOdThreads) void k_od_yfilter(Bases b, const MsgJob *__restrict__ jobs,
                                                            const float *__restrict__ taps, int R) {
   const MsgJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   const int th = J.th, tw = J.tw;
   const long long cells = (long long)th * tw;
   const long long n = R * cells;
@@ -198,6 +203,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_yfilter(Bases b, const MsgJob
 __global__ __launch_bounds__(kOdThreads) void k_od_back(Bases b, const MsgJob *__restrict__ jobs,
                                                         const int *__restrict__ shifts, int R) {
   const MsgJob &J = jobs[blockIdx.y];
+  if (skip_inst(b, J.inst)) return;
   const int H = J.H, W = J.W, th = J.th, tw = J.tw;
   const long long HW = (long long)H * W;
   const long long n = R * HW;

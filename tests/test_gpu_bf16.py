@@ -23,11 +23,15 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_method(model, unaries, joints, method, graph=False):
+def run_method(model, unaries, joints, method, graph=False, guard=True):
+    """With guard (the default) the plan's MAP guard runs at the method's
+    threshold: instances whose MAP certificate fails are swept again in FP32,
+    so every MAP must equal the FP64 oracle's."""
     import torch
     sizes = [u.shape[3:] for u in unaries]
     plan = Plan(model, sizes, joints)
     levels, used = plan.set_method(method)
+    plan.set_map_guard(Plan.MAP_GUARD_TAU[method] if guard else 0.0)
     dev = torch.device("cuda:0")
     un = torch.from_numpy(np.concatenate([u.ravel() for u in unaries])).to(dev)
     marg = torch.full_like(un, float("nan"))
@@ -114,7 +118,7 @@ def test_uneven_scales_masks_and_offdiagonal(gpu, method):
         worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
         say("%s uneven %d: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (method, i, worst, map_bad))
         assert worst <= METHOD_L1[method]
-        assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
+        assert not map_bad, map_bad
 
 
 @pytest.mark.parametrize("method", ["bf16mix", "bf16x2", "bf16"])
@@ -142,10 +146,8 @@ def test_fp16_intermediate_uneven_widths_and_rotation_copy(gpu, method):
         worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
         say("  %s instance %d %s: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (method, i, sizes[i], worst,
                                                                                    map_bad))
-        # a MAP may move between two states whose log scores are closer than
-        # the method's error level (a near tie), nowhere else
         assert worst <= METHOD_L1[method]
-        assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
+        assert not map_bad, map_bad
 
 
 # parts without a detector (is_detect = 0): a leaf whose source is the zero
@@ -174,7 +176,7 @@ def test_parts_without_detector(gpu, case, method):
         worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
         say("  %s %s instance %d: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (case, method, i, worst, map_bad))
         assert worst <= METHOD_L1[method]
-        assert all(m[4] < 2 * METHOD_L1[method] for m in map_bad), map_bad
+        assert not map_bad, map_bad
 
 
 def test_fp16_widest_and_tallest_grids(gpu):
@@ -192,7 +194,7 @@ def test_fp16_widest_and_tallest_grids(gpu):
         worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
         say("  instance %d %s: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (i, sizes[i], worst, map_bad))
         assert worst <= METHOD_L1["bf16mix"]
-        assert all(m[4] < 2 * METHOD_L1["bf16mix"] for m in map_bad), map_bad
+        assert not map_bad, map_bad
 
 
 @pytest.mark.parametrize("method", ["bf16mix", "bf16x3"])
@@ -208,7 +210,10 @@ def test_tiny_grids(gpu, method):
     joints = [synth_joints(40 + i) for i in range(len(sizes))]
     model = person_model(ep, joints[0])
     us = [synth_unary(50 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(sizes)]
-    outs, (levels, used) = run_method(model, us, joints, method)
+    # unguarded: the degenerate instances have near ties everywhere, and the
+    # guard's FP32 re-evaluation reproduces the reference's float32 overflow
+    # there (+inf marginals on the 9 x 11 grid, as the float32 oracle)
+    outs, (levels, used) = run_method(model, us, joints, method, guard=False)
     assert used == levels
     for i, (h, w) in enumerate(sizes):
         marg, root, maps = outs[i]
@@ -218,6 +223,13 @@ def test_tiny_grids(gpu, method):
             assert 0 <= int(m["rotidx"]) < 24 and 0 <= int(m["y"]) < h and 0 <= int(m["x"]) < w
     worst, map_bad = errors_vs_f64(model, us[3], joints[3], outs[3])
     say("%s tiny grids: 20x24 worst rel L1 vs f64 %.3g, MAP mismatches %s" % (method, worst, map_bad))
+    assert worst <= BUDGET_L1 and not map_bad
+    # guarded, the 20 x 24 instance keeps its MAPs and the 9 x 11 one (ties
+    # everywhere) takes the float32 path's outputs
+    outs_g, _ = run_method(model, us, joints, method, guard=True)
+    outs_32, _ = run_method(model, us, joints, "fp32")
+    assert np.array_equal(outs_g[2][0], outs_32[2][0])
+    worst, map_bad = errors_vs_f64(model, us[3], joints[3], outs_g[3])
     assert worst <= BUDGET_L1 and not map_bad
 
 
@@ -261,7 +273,7 @@ def test_width_limit_of_the_mfma_passes(gpu):
     worst, map_bad = errors_vs_f64(model, u512, joints, outs[0])
     say("512 wide: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (worst, map_bad))
     assert worst <= METHOD_L1["bf16mix"]
-    assert all(m[4] < 2 * METHOD_L1["bf16mix"] for m in map_bad), map_bad
+    assert not map_bad, map_bad
     u520 = synth_unary(142, 10, 1, 24, 96, 520)
     outs_mix, (levels, used) = run_method(model, [u520], [joints], "bf16mix")
     assert used == 0 and levels == 4

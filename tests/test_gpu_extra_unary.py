@@ -118,7 +118,9 @@ def test_sweep_with_extra_unaries(gpu):
     v = oracle.add_extra_unary(v, extra, 0.2)
     v[root] = oracle.torso_pos_prior(v[root], params, 0.9)
     dv = d.cpu().numpy()
-    assert ulps(dv, v).max() <= 4
+    # an ulp of difference in an early term can stand for many ulps of a
+    # later sum that cancels towards zero: compare absolutely (values ~10)
+    assert np.abs(dv.astype(np.float64) - v).max() <= 1e-5
     say("sweep on unaries with extra terms")
     # the sweep reads the device's unaries; the oracle the same values
     out = run_plan(model, [dv])[0]

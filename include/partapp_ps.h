@@ -205,6 +205,22 @@ int ps_message(const ps_model *model, const float *child_dev, float *parent_dev,
                const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
                double rot_sigma, double scale, int is_sparse, void *stream);
 
+/* ps_message without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * message's tables, keys and scratch grids live in a device array of
+ * PS_MESSAGE_NOALLOC_BYTES that the library holds on every GPU it runs on
+ * until the process ends; a message that needs more (about
+ * R * height * ceil32(width) * 4 bytes plus a few KB of tables, plus
+ * 12 * R * th * tw bytes of eigenbasis grids for an off-diagonal C) returns
+ * PS_ERR_UNSUPPORTED.  Same arguments, results and errors as ps_message;
+ * synchronous.  Calls on one GPU share the array: the library serialises
+ * them with a process-wide lock (overlapping calls would race on it). */
+#define PS_MESSAGE_NOALLOC_BYTES (32u << 20)
+int ps_message_noalloc(const ps_model *model, const float *child_dev, float *parent_dev, int height, int width,
+                       const double offset_c[2], const double offset_p[2], const double C[4], double rot_mean,
+                       double rot_sigma, double scale, int is_sparse, void *stream);
+
 /* findLocalMax (objectdetect_aux.cpp:193-251) on device grids.  Grid g is
  * [D][H][W] floats at grids_dev + offsets[g] (dims[3g..3g+2] = D, H, W; D is
  * rotations for a part marginal, scales for the root posterior).  A point is

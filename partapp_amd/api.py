@@ -349,19 +349,22 @@ def compute_root_posterior_rot(model: PartModel, log_part_detections, joints: Op
 
 
 def compute_rot_joint_marginal(model: PartModel, log_prob_child, offset_c, offset_p, C, rot_mean, rot_sigma,
-                               scale, is_sparse=False, stream=None):
+                               scale, is_sparse=False, stream=None, out=None, no_alloc=False):
     """computeRotJointMarginal (objectdetect_findrot.cpp:292-456): child [R][H][W]
-    (torch CUDA float32) -> parent message; is_sparse is bIsSparse."""
+    (torch CUDA float32) -> parent message; is_sparse is bIsSparse.  `out`:
+    the parent tensor to write (default: a new one).  no_alloc: run through
+    ps_message_noalloc (no device allocation inside the library)."""
     import torch
     R, H, W = log_prob_child.shape
     child = log_prob_child.contiguous()
-    parent = torch.empty_like(child)
+    parent = torch.empty_like(child) if out is None else out
+    assert parent.is_contiguous() and parent.shape == child.shape and parent.dtype == torch.float32
     oc = (ctypes.c_double * 2)(*offset_c)
     op = (ctypes.c_double * 2)(*offset_p)
     cc = (ctypes.c_double * 4)(*C)
-    check(lib().ps_message(model.handle, child.data_ptr(), parent.data_ptr(), H, W, oc, op, cc, float(rot_mean),
-                           float(rot_sigma), float(scale), 1 if is_sparse else 0, _stream_handle(stream)),
-          "ps_message")
+    fn, name = (lib().ps_message_noalloc, "ps_message_noalloc") if no_alloc else (lib().ps_message, "ps_message")
+    check(fn(model.handle, child.data_ptr(), parent.data_ptr(), H, W, oc, op, cc, float(rot_mean), float(rot_sigma),
+             float(scale), 1 if is_sparse else 0, _stream_handle(stream)), name)
     return parent
 
 

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libpartapp_ps.so"
 
-SOURCES = ["ps_api.hip", "ps_kernels.hip", "ps_offdiag.hip", "ps_localmax.hip", "ps_ingest.hip", "ps_mfma.hip", "ps_plan.cpp", "ps_synth.cpp", "ps_synth_dev.hip", "ps_matio.cpp"]
+SOURCES = ["ps_api.hip", "ps_kernels.hip", "ps_offdiag.hip", "ps_localmax.hip", "ps_ingest.hip", "ps_mfma.hip", "ps_plan.cpp", "ps_synth.cpp", "ps_synth_dev.hip", "ps_matio.cpp", "ps_noalloc.hip"]
 
 # -packed-fp32-ops: no v_pk_*_f32 from the compiler.  -ffp-contract=off: the
 # only fused multiply-adds are the explicit fmaf() of the filters, and the

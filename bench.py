@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-guard", action="store_true", help="skip the MAP-guarded run of the chosen method")
+    ap.add_argument("--no-prune", action="store_true", help="skip the learned-pruning run of the chosen method")
     ap.add_argument("--graph", action="store_true", help="time HIP-graph replays (no per-kernel roofline)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 256 MiB (the known-size kernel that scales the rocprofv3 "
@@ -252,7 +253,7 @@ def time_method(plan, method, unary, marg, root, maps, stream, args, steps=None,
     return t0.elapsed_time(t1), prof
 
 
-def check_instances(plan, model, unary, legs, ep, joints, idx):
+def check_instances(plan, model, unary, legs, ep, joints, idx, pruned=None):
     """Error of each leg's last sweep on the instances `idx`, with the very
     unaries the GPU read.  legs: {name: (marginal tensor, map tensor)}; the
     leg named "fp32" is held to the FP32 bar against the float32 oracle (the
@@ -279,10 +280,19 @@ def check_instances(plan, model, unary, legs, ep, joints, idx):
         if "fp32" in legs:
             _, p32, b32, _ = oracle.root_posterior_rot(oep, model.is_detect, model.is_upright, model.root,
                                                        joints[i], u)
-        out = {k: {"l1_64": 0.0, "l1_32": 0.0, "agree64": 0, "agree32": 0, "ties": [], "far": []} for k in legs}
+        out = {k: {"l1_64": 0.0, "l1_32": 0.0, "agree64": 0, "agree32": 0, "ties": [], "far": [], "drop": 0.0}
+               for k in legs}
+        if pruned is not None:  # the FP64 mass of the pruned leg's dropped tiles (none for a fallback instance)
+            name, keep_h, offs, fb_h = pruned
+            H, W = plan.sizes[i]
+            ty, tx = (H + 63) // 64, (W + 63) // 64
+            kp = keep_h[offs[i]:offs[i] + plan.P * plan.S * plan.R * ty * tx].reshape(plan.P, plan.S, plan.R, ty, tx)
         for s in range(plan.S):
             for p in range(plan.P):
                 n64 = oracle.normalized(p64[p, s])
+                if pruned is not None and not fb_h[i]:
+                    kf = np.repeat(np.repeat(kp[p, s], 64, axis=1), 64, axis=2)[:, :H, :W].astype(bool)
+                    out[name]["drop"] += float(n64[~kf].sum())
                 n32 = oracle.normalized(p32[p, s]) if "fp32" in legs else None
                 for k in legs:
                     r = out[k]
@@ -312,6 +322,11 @@ def check_instances(plan, model, unary, legs, ep, joints, idx):
                   "map_mismatches": [[int(a), int(b), int(c), float("%.3g" % g)] for a, b, c, g in (ties + far)[:8]],
                   "sample": "%d of %d instances (%d (instance, scale, part) MAPs)" % (len(idx), plan.n_instances,
                                                                                      total)}
+        if pruned is not None and k == pruned[0]:
+            drops = [r[k]["drop"] for r in per]
+            res[k]["fp64_mass_of_dropped_tiles"] = {"max": float("%.3g" % max(drops)),
+                                                    "median": float("%.3g" % float(np.median(drops)))}
+            res[k]["bar"] = "rel L1 <= 1e-3 vs the FP64 oracle"
         if k == "fp32":
             res[k]["max_rel_l1_vs_f32_oracle"] = float("%.3g" % max(r[k]["l1_32"] for r in per))
             res[k]["map_identical_vs_f32_oracle"] = "%d/%d" % (sum(r[k]["agree32"] for r in per), total)
@@ -438,17 +453,65 @@ def main():
         print("%s with the MAP guard (tau %g): %.3f ms/step, %d of %d instances re-evaluated" % (
             args.method, tau, msg_ / args.steps, n_re, n), file=sys.stderr, flush=True)
 
+    # learned pruning: the net's keep masks (and fallback flags) recomputed
+    # from the unaries inside every timed step, then the pruned sweep
+    pruned, prune_info = None, None
+    if not args.no_prune and args.method in ("bf16", "bf16x2", "bf16mix"):
+        from partapp_amd import prune
+        if os.path.exists(prune.WEIGHTS):
+            pr = prune.Pruner(plan, dev)
+            margp = torch.empty_like(unary)
+            mapsp = torch.zeros_like(maps)
+            plan.set_method(args.method)
+            plan.set_map_guard(0.0)
+            with torch.cuda.stream(stream):
+                for _ in range(args.warmup):
+                    pr.update(unary)
+                    plan.execute_pruned(unary, margp, root, mapsp, pr.keep, pr.fallback, stream=stream)
+                torch.cuda.synchronize()
+                t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t0.record(stream)
+                for _ in range(args.steps):
+                    pr.update(unary)
+                    plan.execute_pruned(unary, margp, root, mapsp, pr.keep, pr.fallback, stream=stream)
+                t1.record(stream)
+                torch.cuda.synchronize()
+            msp = t0.elapsed_time(t1)
+            # the net and the keep rule alone (features, convolutions, flags)
+            with torch.cuda.stream(stream):
+                t2, t3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t2.record(stream)
+                for _ in range(args.steps):
+                    pr.update(unary)
+                t3.record(stream)
+                torch.cuda.synchronize()
+            ms_net = t2.elapsed_time(t3) / args.steps
+            fb = pr.fallback.cpu().numpy()
+            pruned = {"value": round(n * args.steps / (msp / 1e3), 3), "ms_per_step": round(msp / args.steps, 4),
+                      "net_and_masks_ms_per_step": round(ms_net, 4),
+                      "kept_tiles": round(pr.kept_fraction(), 4), "fallback_rate": "%d/%d" % (int(fb.sum()), n),
+                      "predicted_dropped_mass_max": float("%.3g" % float(pr.predicted_dropped.max())),
+                      "net": "3x3 convs 240->64->32->32, 1x1 -> 240, BF16 weights, on 4x max-pooled unaries; "
+                             "per part, 64x64 tiles kept in decreasing predicted mass until at most 1e-4 is left "
+                             "out; fall back to the exact sweep above 1e-3 predicted dropped mass per instance"}
+            legs[args.method + "+prune"] = (margp, mapsp)
+            prune_info = (args.method + "+prune", pr.keep.cpu().numpy(), pr.offsets, fb)
+            print("%s with learned pruning: %.3f ms/step, kept tiles %.3f, fallback %d/%d" % (
+                args.method, msp / args.steps, pr.kept_fraction(), int(fb.sum()), n), file=sys.stderr, flush=True)
+
     err = None
     k = n if args.check is None else min(args.check, n)
     if k > 0:
         legs[args.method] = (marg, maps)
         tc = time.perf_counter()
-        chk = check_instances(plan, model, unary, legs, ep, joints, list(range(k)))
+        chk = check_instances(plan, model, unary, legs, ep, joints, list(range(k)), pruned=prune_info)
         err = chk[args.method]
         if fp32 is not None:
             fp32["parity"] = chk["fp32"]
         if guarded is not None:
             guarded["error_vs_fp64"] = chk[args.method + "+guard"]
+        if pruned is not None:
+            pruned["error_vs_fp64"] = chk[args.method + "+prune"]
         print("checked %d instances in %.1f s: %s" % (k, time.perf_counter() - tc, json.dumps(chk)), file=sys.stderr,
               flush=True)
 
@@ -541,6 +604,7 @@ def main():
         "plain_fp32": fp32,
         "vs_plain_fp32": round(value / fp32["value"], 3) if fp32 else None,
         "map_guard": guarded,
+        "learned_pruning": pruned,
         "error_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,

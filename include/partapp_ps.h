@@ -173,6 +173,23 @@ int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
  * tau = 0 turns the guard off (the default).  ps_plan_read_map_guard
  * reports how many instances the last execution re-evaluated (and, if
  * flags_host is given, which: [n_instances]); it waits for `stream`. */
+/* Pruned execution (no counterpart in the reference).  keep_dev holds one
+ * byte per 64 x 64 tile of every (instance, part, scale, rotation) plane:
+ * instance i's flags start at byte instance_offsets[i] of
+ * ps_plan_keep_layout and are laid out [P][S][R][ceil(H/64)][ceil(W/64)].
+ * In the BF16 methods' column pass a destination tile whose flag is 0 is
+ * not computed and is written as LOG_ZERO: the sweep then runs on the
+ * pruned state space (zero probability outside the kept tiles), whose
+ * marginals differ from the exact ones by at most twice the probability mass
+ * the exact marginals put on the dropped tiles.  Levels that run the FP32
+ * passes compute every tile.  fallback_dev (optional, [n_instances]): an
+ * instance with a nonzero flag runs the whole (unpruned) sweep instead.
+ * Asynchronous; the MAP guard is not applied. */
+int ps_plan_keep_layout(const ps_plan *plan, long long *instance_offsets, long long *total_bytes);
+int ps_plan_execute_pruned(ps_plan *plan, const float *log_unary, float *log_marginal, float *root_posterior,
+                           ps_map *map_dev, const unsigned char *keep_dev, const unsigned *fallback_dev,
+                           void *stream);
+
 int ps_plan_set_map_guard(ps_plan *plan, float tau);
 int ps_plan_read_map_guard(ps_plan *plan, int *n_reevaluated, unsigned *flags_host, void *stream);
 

@@ -108,6 +108,10 @@ SIGNATURES = {
                                              ctypes.c_void_p, ctypes.c_void_p]),
     "ps_plan_set_method": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_plan_method_levels": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p]),
+    "ps_plan_keep_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                           ctypes.POINTER(ctypes.c_longlong)]),
+    "ps_plan_execute_pruned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ps_plan_set_map_guard": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
     "ps_plan_read_map_guard": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.POINTER(ctypes.c_uint),
                                               ctypes.c_void_p]),

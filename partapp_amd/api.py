@@ -210,6 +210,23 @@ class Plan:
         check(fn(self.handle, _ptr(log_unary), _ptr(log_marginal), _ptr(root_posterior), _ptr(map_dev),
                  _stream_handle(stream)), "ps_plan_execute")
 
+    def keep_layout(self):
+        """(per-instance byte offsets, total bytes) of the keep flags of a
+        pruned execution: one byte per 64 x 64 tile, [P][S][R][ty][tx] per
+        instance."""
+        offs = (ctypes.c_longlong * self.n_instances)()
+        total = ctypes.c_longlong()
+        check(lib().ps_plan_keep_layout(self.handle, offs, ctypes.byref(total)), "ps_plan_keep_layout")
+        return list(offs), total.value
+
+    def execute_pruned(self, log_unary, log_marginal, root_posterior, map_dev, keep, fallback=None, stream=None):
+        """Pruned sweep: tiles whose keep byte is 0 are not computed (LOG_ZERO);
+        instances with a nonzero fallback flag (uint32 per instance) run the
+        whole sweep instead."""
+        check(lib().ps_plan_execute_pruned(self.handle, _ptr(log_unary), _ptr(log_marginal), _ptr(root_posterior),
+                                           _ptr(map_dev), _ptr(keep), _ptr(fallback), _stream_handle(stream)),
+              "ps_plan_execute_pruned")
+
     KERNEL_CLASSES = ("reset", "leaf_max", "rowpass", "colpass", "root", "map_decode")
 
     METHODS = {"fp32": 0, "bf16": 1, "bf16x2": 2, "bf16x3": 3, "bf16mix": 4}

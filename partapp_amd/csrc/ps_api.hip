@@ -24,6 +24,7 @@ namespace ps {
 extern thread_local std::string g_last_error;
 hipError_t launch_reset(const Bases &b, int n_keys, int n_map, int n_inst, unsigned *flags, int n_flags,
                         hipStream_t s);
+hipError_t launch_invert_flags(const unsigned *in, unsigned *out, int n, hipStream_t s);
 hipError_t launch_certify(const unsigned long long *top2, int n_map, int n_inst, float tau, unsigned *flags,
                           hipStream_t s);
 hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int max_hw, hipStream_t s);
@@ -155,6 +156,10 @@ struct ps_plan {
   unsigned long long *top2 = nullptr;
   unsigned *flags = nullptr;
   float guard_tau = 0.0f;
+  // pruned execution: keep flags per 64 x 64 tile, [inst][P][S][R][ty][tx]
+  std::vector<long long> keep_offsets;  // per instance, bytes
+  long long keep_bytes = 0;
+  unsigned *not_fallback = nullptr;     // per instance: 1 - fallback flag (the pruned pass's active set)
   std::vector<Level> up, down;
   int n_roots = 0, n_maxes = 0;
   int root_children = 0;  // the same for every root job
@@ -438,6 +443,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   const int items = n_instances * S;
   std::vector<std::vector<MsgJob>> up_jobs(t.max_height), dn_jobs(t.max_depth);
   size_t scratch_off = 0;
+  std::vector<long long> keep_offsets(n_instances);
+  long long keep_total = 0;
   int max_rows = 0, max_hw = 0;
   for (int i = 0; i < n_instances; ++i) {
     const ps_instance &in = instances[i];
@@ -446,6 +453,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     const ps_joint *J = in.joints ? in.joints : m.joints.data();
     max_rows = std::max(max_rows, R * H);
     max_hw = std::max(max_hw, H * W);
+    const int kty = (H + 63) / 64, ktx = (W + 63) / 64;
+    keep_offsets[i] = keep_total;
+    keep_total += (long long)P * S * R * kty * ktx;
     for (int s = 0; s < S; ++s) {
       const int it = i * S + s;
       const double scale = scale_from_index(m.ep, s);
@@ -458,6 +468,11 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       auto up_key = [&](int p) { return it * 2 * P + p; };
       auto dn_key = [&](int p) { return it * 2 * P + P + p; };
       auto slot = [&](int p) { return it * P + p; };
+      auto keep_plane = [&](MsgJob &mj, int p) {
+        mj.keep_off = keep_offsets[i] + ((long long)p * S + s) * R * kty * ktx;
+        mj.keep_ty = kty;
+        mj.keep_tx = ktx;
+      };
 
       // leaf maxima
       for (int k = 0; k < K; ++k) {
@@ -535,6 +550,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           mj.tmp = tbuf(k);
           mj.dst2 = kNone;
           mj.dst_unary = kNone;
+          keep_plane(mj, p);
           if (p == t.root) {
             mj.dst_kind = kDstUpRoot;
             mj.dst = mbuf(k);
@@ -598,6 +614,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           mj.tmp = tbuf(k);
           mj.dst_kind = kDstDown;
           mj.dst = marg(c);
+          keep_plane(mj, c);
           mj.dst_unary = unary(c);
           mj.dst_mask = part_mask(m, c, W);
           const bool has_child = t.child_joint[c] >= 0;
@@ -636,6 +653,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->items = items;
   pl->max_rows = max_rows;
   pl->max_hw = max_hw;
+  pl->keep_offsets = keep_offsets;
+  pl->keep_bytes = keep_total;
   auto add_levels = [&](std::vector<std::vector<MsgJob>> &lv, std::vector<Level> &dst) -> int {
     for (auto &v : lv) {
       std::stable_partition(v.begin(), v.end(), [](const MsgJob &j) { return j.od == 0; });
@@ -728,7 +747,8 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   size_t o_mkeys = align(o_keys + (size_t)pl->n_keys * sizeof(unsigned));
   size_t o_top2 = align(o_mkeys + (size_t)pl->n_map * sizeof(unsigned long long));
   size_t o_flags = align(o_top2 + (size_t)pl->n_map * sizeof(unsigned long long));
-  size_t total = align(o_flags + (size_t)n_instances * sizeof(unsigned));
+  size_t o_nfb = align(o_flags + (size_t)n_instances * sizeof(unsigned));
+  size_t total = align(o_nfb + (size_t)n_instances * sizeof(unsigned));
   hipError_t e = hipMalloc(&pl->tables, total);
   if (e != hipSuccess) {
     delete pl;
@@ -746,6 +766,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->map_keys = reinterpret_cast<unsigned long long *>(base + o_mkeys);
   pl->top2 = reinterpret_cast<unsigned long long *>(base + o_top2);
   pl->flags = reinterpret_cast<unsigned *>(base + o_flags);
+  pl->not_fallback = reinterpret_cast<unsigned *>(base + o_nfb);
   std::vector<char> host(total, 0);
   std::memcpy(host.data() + o_jobs, B.jobs.data(), B.jobs.size() * sizeof(MsgJob));
   std::memcpy(host.data() + o_roots, B.roots.data(), B.roots.size() * sizeof(RootJob));
@@ -991,7 +1012,37 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
 // its second best by at most guard_tau) is swept again in FP32, on the
 // device, in the same stream order.
 static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, float *root_posterior, ps_map *map_dev,
-                   hipStream_t s, bool profile) {
+                   hipStream_t s, bool profile, const unsigned char *keep = nullptr,
+                   const unsigned *fallback = nullptr) {
+  if (keep) {
+    // a pruned execution: the instances without the fallback flag run on
+    // their kept tiles, then the flagged ones run the whole sweep
+    Bases b;
+    b.unary = log_unary;
+    b.marginal = log_marginal;
+    b.scratch = pl->scratch;
+    b.root = root_posterior;
+    b.max_keys = pl->keys;
+    b.map_keys = pl->map_keys;
+    b.top2_keys = nullptr;
+    b.active = nullptr;
+    b.keep = nullptr;
+    ps_plan *pp = profile ? pl : nullptr;
+    hipError_t e = hipSuccess;
+    if (fallback) {
+      e = launch_invert_flags(fallback, pl->not_fallback, pl->n_inst, s);
+      b.active = pl->not_fallback;
+    }
+    b.keep = keep;
+    if (e == hipSuccess) e = sweep(pl, b, pl->terms, pl->col_terms, map_dev, s, pp, nullptr);
+    if (e == hipSuccess && fallback) {
+      b.keep = nullptr;
+      b.active = fallback;
+      e = sweep(pl, b, pl->terms, pl->col_terms, map_dev, s, pp, nullptr);
+    }
+    if (e != hipSuccess) return hip_error(e, "kernel launch");
+    return PS_OK;
+  }
   const bool guard = pl->guard_tau > 0.0f && pl->terms != 0;
   Bases b;
   b.unary = log_unary;
@@ -1002,6 +1053,7 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
   b.map_keys = pl->map_keys;
   b.top2_keys = guard ? pl->top2 : nullptr;
   b.active = nullptr;
+  b.keep = nullptr;
   ps_plan *pp = profile ? pl : nullptr;
   hipError_t e = sweep(pl, b, pl->terms, pl->col_terms, map_dev, s, pp, pl->flags);
   if (e == hipSuccess && guard) {
@@ -1032,6 +1084,24 @@ extern "C" int ps_plan_execute(ps_plan *plan, const float *log_unary, float *log
   if (st != PS_OK) return st;
   return enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, static_cast<hipStream_t>(stream),
                  plan->profile_mask != 0u);
+}
+
+extern "C" int ps_plan_keep_layout(const ps_plan *plan, long long *instance_offsets, long long *total_bytes) {
+  if (!plan || !total_bytes) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  *total_bytes = plan->keep_bytes;
+  if (instance_offsets)
+    for (int i = 0; i < plan->n_inst; ++i) instance_offsets[i] = plan->keep_offsets[i];
+  return PS_OK;
+}
+
+extern "C" int ps_plan_execute_pruned(ps_plan *plan, const float *log_unary, float *log_marginal,
+                                      float *root_posterior, ps_map *map_dev, const unsigned char *keep_dev,
+                                      const unsigned *fallback_dev, void *stream) {
+  int st = check_exec_args(plan, log_unary, log_marginal);
+  if (st != PS_OK) return st;
+  if (!keep_dev) return set_error(PS_ERR_INVALID_ARG, "keep_dev is required");
+  return enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, static_cast<hipStream_t>(stream),
+                 plan->profile_mask != 0u, keep_dev, fallback_dev);
 }
 
 extern "C" int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, float *log_marginal,
@@ -1186,6 +1256,7 @@ int message_enqueue(const MessageSetup &ms, char *mem, const float *child_dev, f
   b.map_keys = reinterpret_cast<unsigned long long *>(mem + ms.o_mk);
   b.top2_keys = nullptr;
   b.active = nullptr;
+  b.keep = nullptr;
   const MsgJob *jd = reinterpret_cast<const MsgJob *>(mem + ms.o_job);
   const float *td = reinterpret_cast<const float *>(mem + ms.o_taps);
   const int *sd = reinterpret_cast<const int *>(mem + ms.o_sh);

@@ -33,6 +33,8 @@ struct Bases {
   unsigned long long *map_keys;  // (score, first index) argmax keys
   unsigned long long *top2_keys; // per argmax key: (best value key << 32) | second best value key, or null
   const unsigned *active;        // per instance: 0 = skip it in this execution; null = run every instance
+  const unsigned char *keep;     // pruned execution: keep flag per 64 x 64 tile of every (part, rotation)
+                                 // plane (MsgJob::keep_off); null = every tile
 };
 
 // Upright / border masking of one part's unaries
@@ -99,6 +101,10 @@ struct MsgJob {
   int taps_xe;     // offsets into the tap pool
   int taps_ye;
   GridRef od_buf;  // floats: A[R][th][tw] (scatter keys: 2 floats per cell), then X[R][th][tw]
+  // pruned execution: the destination part's keep flags, [R][keep_ty][keep_tx]
+  // tiles of 64 x 64 starting at keep_off (bytes) in Bases::keep
+  long long keep_off;
+  int keep_ty, keep_tx;
   double t_fwd[6];   // rows 0-1; direct: grid -> eigenbasis, bilinear: eigenbasis -> grid
   double t_back[6];  // rows 0-1; grid pixel -> eigenbasis sample position
 };

@@ -671,7 +671,18 @@ __global__ void k_certify(const unsigned long long *__restrict__ top2, int n_map
   if (!(v1 - v2 > tau)) flags[i / map_per_inst] = 1u;
 }
 
+__global__ void k_invert_flags(const unsigned *__restrict__ in, unsigned *__restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] ? 0u : 1u;
+}
+
 // ------------------------------------------------------------ host launchers
+
+hipError_t launch_invert_flags(const unsigned *in, unsigned *out, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_invert_flags, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_certify(const unsigned long long *top2, int n_map, int n_inst, float tau, unsigned *flags,
                           hipStream_t s) {

@@ -569,7 +569,11 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
 // at KK = 7 where per-lane fragments take 14 KB), so that LDS no longer
 // limits a CU to 4 blocks.  Block shape, staging order and epilogue as in
 // colpass_mfma_strip.
-template <int EPI>
+// PRUNE: a pruned execution; a 64 x 64 destination tile whose keep flag is 0
+// skips its GEMM and epilogue operands and is written as LOG_ZERO (zero
+// probability: the pruned model sets those states' unaries to -infinity).
+// The ring keeps staging every tile, so the next kept tile finds its rows.
+template <int EPI, bool PRUNE>
 __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
                                                   const int *__restrict__ shifts, int R, int r, int xt, uint4 *lds) {
   constexpr bool kNeedU = Epi<EPI>::kNeedU;
@@ -665,12 +669,14 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
 
   Epi<EPI> acc;
   int base = 0;
+  const unsigned char *kp = PRUNE ? b.keep + J.keep_off + (long long)r * J.keep_ty * J.keep_tx + xt : nullptr;
   for (int ty0 = 0; ty0 < H; ty0 += 64) {
     asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
     const int yl = ty0 + ry + 4 * h;
     const unsigned eb0 = (unsigned)(yl * W + xo) * 4u;
+    const bool kept = !PRUNE || kp[(ty0 >> 6) * J.keep_tx] != 0;
     float pu[16], pb[16];
-    if (kNeedU || kNeedB) {
+    if ((kNeedU || kNeedB) && kept) {
       if (ty0 + 64 <= H) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -692,25 +698,37 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
     h2_t pre[8];
     load16(pre_row, pre);
 
-    f32x16_t c = {};
-    int lrow = base + ry + kq;
-    if (lrow >= nrows) lrow -= nrows;
-    for (int kk = 0; kk < KK; ++kk) {
-      const f16x8_t bv = *reinterpret_cast<const f16x8_t *>(b_col + lrow);
-      c = mfma16(*reinterpret_cast<const f16x8_t *>(fa + 2 * kk), bv, c);
-      c = mfma16(*reinterpret_cast<const f16x8_t *>(fa + 8 * q8 + 2 * kk), bv, c);
-      lrow += 16;
+    if (kept) {
+      f32x16_t c = {};
+      int lrow = base + ry + kq;
       if (lrow >= nrows) lrow -= nrows;
-    }
+      for (int kk = 0; kk < KK; ++kk) {
+        const f16x8_t bv = *reinterpret_cast<const f16x8_t *>(b_col + lrow);
+        c = mfma16(*reinterpret_cast<const f16x8_t *>(fa + 2 * kk), bv, c);
+        c = mfma16(*reinterpret_cast<const f16x8_t *>(fa + 8 * q8 + 2 * kk), bv, c);
+        lrow += 16;
+        if (lrow >= nrows) lrow -= nrows;
+      }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int dy = 8 * (i >> 2) + (i & 3);
-      const int y = yl + dy;
-      const float lg = __builtin_amdgcn_logf(c[i]);  // -inf for 0
-      const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
-      const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
-      if (xin && y < H)
-        acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask, leaf_u);
+      for (int i = 0; i < 16; ++i) {
+        const int dy = 8 * (i >> 2) + (i & 3);
+        const int y = yl + dy;
+        const float lg = __builtin_amdgcn_logf(c[i]);  // -inf for 0
+        const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
+        const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
+        if (xin && y < H)
+          acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask,
+                    leaf_u);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int dy = 8 * (i >> 2) + (i & 3);
+        if (xin && yl + dy < H) {
+          at(dst, eb0 + (unsigned)dy * Wb) = kLogZero;
+          if (kNeedB) at(dst2, eb0 + (unsigned)dy * Wb) = kLogZero;
+        }
+      }
     }
     if (more) {
       __syncthreads();  // every wave is done with this tile's ring rows
@@ -729,7 +747,7 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
 // only the kinds up to HI, so a launch without the five-grid downward
 // epilogue is not held to its register count (UpRoot 86, UpNodeU 94,
 // DownLeaf 98, DownMid 116 VGPRs + AGPRs).
-template <int HI>
+template <int HI, bool PRUNE>
 __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__restrict__ jobs,
                                                      const uint4 *__restrict__ frags, const int *__restrict__ shifts,
                                                      int R, int xtiles) {
@@ -743,15 +761,15 @@ __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__re
   if (xt * 64 >= J.W) return;
   const int kind = epi_kind(J);
   if (HI == kEpiUpRoot || kind == kEpiUpRoot)
-    colpass_f16_strip<kEpiUpRoot>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiUpRoot, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
   else if (HI == kEpiUpNode || kind == kEpiUpNode)
-    colpass_f16_strip<kEpiUpNode>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiUpNode, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
   else if (HI == kEpiUpNodeU || kind == kEpiUpNodeU)
-    colpass_f16_strip<kEpiUpNodeU>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiUpNodeU, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
   else if (HI == kEpiDownLeaf || kind == kEpiDownLeaf)
-    colpass_f16_strip<kEpiDownLeaf>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiDownLeaf, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
   else
-    colpass_f16_strip<kEpiDownMid>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiDownMid, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
 }
 
 template <int TERMS>
@@ -801,11 +819,16 @@ hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, false>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode, false>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU, false>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf, false>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf, true>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid, false>), col_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid, true>), col_bytes);
   return e;
 }
 
@@ -836,13 +859,21 @@ hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
   const int xtiles = (max_w + 63) / 64;
   dim3 grid(R * xtiles, n_jobs);
   const dim3 blk(256);
-  if (t16) {  // fp16 operands: exact data, split taps, whatever the method's terms
+  if (t16 && b.keep) {  // a pruned execution
     switch (epi_hi) {
-      case kEpiUpRoot: hipLaunchKernelGGL(k_colpass_f16<kEpiUpRoot>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiUpNode: hipLaunchKernelGGL(k_colpass_f16<kEpiUpNode>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiUpNodeU: hipLaunchKernelGGL(k_colpass_f16<kEpiUpNodeU>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiDownLeaf: hipLaunchKernelGGL(k_colpass_f16<kEpiDownLeaf>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      default: hipLaunchKernelGGL(k_colpass_f16<kEpiDownMid>, grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpRoot: hipLaunchKernelGGL((k_colpass_f16<kEpiUpRoot, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpNode: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNode, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNodeU, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass_f16<kEpiDownLeaf, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      default: hipLaunchKernelGGL((k_colpass_f16<kEpiDownMid, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+    }
+  } else if (t16) {  // fp16 operands: exact data, split taps, whatever the method's terms
+    switch (epi_hi) {
+      case kEpiUpRoot: hipLaunchKernelGGL((k_colpass_f16<kEpiUpRoot, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpNode: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNode, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNodeU, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass_f16<kEpiDownLeaf, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+      default: hipLaunchKernelGGL((k_colpass_f16<kEpiDownMid, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
     }
   } else  // an FP32 row-pass output (3-term row pass): bf16 hi / lo split of data and taps
     hipLaunchKernelGGL(k_colpass_mfma<3>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);

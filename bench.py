@@ -409,22 +409,25 @@ def main():
 
     fp32 = None
     legs = {}
+    tuned = {}
     if not args.no_fp32 and args.method != "fp32":
         # the FP32 leg writes its own outputs, so both legs can be checked
         marg32 = torch.empty_like(unary)
         maps32 = torch.zeros_like(maps)
+        if not args.no_autotune:
+            plan.set_method("fp32")
+            tuned["fp32"] = plan.autotune(unary, marg32, root, maps32, method_is_fp32=True, stream=stream)
+            print("autotune fp32: %s" % tuned["fp32"], file=sys.stderr, flush=True)
         ms32, _ = time_method(plan, "fp32", unary, marg32, root, maps32, stream, args)
         legs["fp32"] = (marg32, maps32)
         fp32 = {"value": round(n * args.steps / (ms32 / 1e3), 3), "ms_per_step": round(ms32 / args.steps, 4)}
         print("plain fp32: %.3f ms/step" % (ms32 / args.steps), file=sys.stderr, flush=True)
 
-    tuned = None
-    if args.method != "fp32" and not args.no_autotune:
+    if not args.no_autotune:
         plan.set_method(args.method)
-        times = plan.autotune(unary, marg, root, maps, stream=stream)
-        tuned = {"mfma_rows_per_block": min(times, key=times.get),
-                 "ms_per_sweep": {str(k): round(v, 3) for k, v in times.items()}}
-        print("autotune: %s" % tuned, file=sys.stderr, flush=True)
+        tuned[args.method] = plan.autotune(unary, marg, root, maps, method_is_fp32=args.method == "fp32",
+                                           stream=stream)
+        print("autotune %s: %s" % (args.method, tuned[args.method]), file=sys.stderr, flush=True)
     # per-kernel breakdown from a separate profiled pass (every class bracketed
     # by events), then the timed steps with events around the dominant kernel
     # only, so that the events barely perturb the measured throughput
@@ -600,7 +603,8 @@ def main():
                    "width": W if args.config == 1 else "160..480", "rotations": R, "scales": ep.num_scale_steps,
                    "states_per_part": states,
                    "parts": 10, "method": args.method, "execution": "hipgraph" if args.graph else "stream",
-                   "autotune": tuned},
+                   "autotune": tuned or None,
+                   "launch_classes": plan.classes()},
         "plain_fp32": fp32,
         "vs_plain_fp32": round(value / fp32["value"], 3) if fp32 else None,
         "map_guard": guarded,

@@ -157,10 +157,20 @@ enum {
 };
 int ps_plan_set_method(ps_plan *plan, int method);
 /* Launch-shape knobs for an autotuner (results do not depend on them):
- * PS_TUNE_MFMA_ROWS = output rows per block of the BF16 x pass (1..64,
- * default 4).  No reference counterpart. */
-enum { PS_TUNE_MFMA_ROWS = 1 };
+ *   PS_TUNE_MFMA_ROWS        output rows per block of the BF16 x pass (1..64,
+ *                            default 4);
+ *   PS_TUNE_FP32_ROW_THREADS threads per block of the FP32 row pass (a
+ *                            multiple of 64 up to 512; 0 = one per column,
+ *                            the default).
+ * ps_plan_tune sets a knob for every launch class; ps_plan_tune_class for
+ * one class.  A class is one (level, width class) row-pass launch of the
+ * sweep, in execution order: ps_plan_classes returns their count and, per
+ * class, (level, jobs, tallest grid, widest grid) in info[4 * c ..].  No
+ * reference counterpart. */
+enum { PS_TUNE_MFMA_ROWS = 1, PS_TUNE_FP32_ROW_THREADS = 2 };
 int ps_plan_tune(ps_plan *plan, int knob, int value);
+int ps_plan_tune_class(ps_plan *plan, int knob, int cls, int value);
+int ps_plan_classes(const ps_plan *plan, int *n_classes, int *info, int max_classes);
 int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
 
 /* MAP guard of the BF16 methods (no counterpart in the reference, whose MAP
@@ -170,6 +180,10 @@ int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels);
  * than tau (twice the method's bound on the error of a log-marginal value)
  * is swept again with the FP32 passes, on the device, before the execution
  * ends (its marginals, root posterior and MAP records then come from FP32).
+ * With ps_plan_execute the flags are read back after the certificate (the
+ * call then waits for the stream) and only the flagged instances' jobs are
+ * launched again; under ps_plan_execute_graph the flags stay on the device
+ * and the FP32 launches cover the batch, their other blocks exiting at once.
  * tau = 0 turns the guard off (the default).  ps_plan_read_map_guard
  * reports how many instances the last execution re-evaluated (and, if
  * flags_host is given, which: [n_instances]); it waits for `stream`. */

@@ -116,6 +116,8 @@ SIGNATURES = {
     "ps_plan_read_map_guard": (ctypes.c_int, [ctypes.c_void_p, c_int_p, ctypes.POINTER(ctypes.c_uint),
                                               ctypes.c_void_p]),
     "ps_plan_tune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "ps_plan_tune_class": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ps_plan_classes": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, ctypes.c_int]),
     "ps_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_plan_set_profiling_classes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "ps_plan_read_profile": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_int_p]),

@@ -261,31 +261,76 @@ class Plan:
         return n.value, [int(f) for f in flags]
 
     TUNE_MFMA_ROWS = 1
+    TUNE_FP32_ROW_THREADS = 2
 
-    def tune(self, knob, value):
-        check(lib().ps_plan_tune(self.handle, knob, int(value)), "ps_plan_tune")
+    def tune(self, knob, value, cls=None):
+        """Set a launch-shape knob for every launch class, or for one."""
+        if cls is None:
+            check(lib().ps_plan_tune(self.handle, knob, int(value)), "ps_plan_tune")
+        else:
+            check(lib().ps_plan_tune_class(self.handle, knob, int(cls), int(value)), "ps_plan_tune_class")
 
-    def autotune(self, unary, marg, root, maps, candidates=(2, 4, 8, 16), reps=2, stream=None):
-        """Pick the fastest rows-per-block of the BF16 x pass for this plan
-        and method by timing `reps` sweeps per candidate (HIP events on the
-        execution stream); sets it and returns {candidate: ms per sweep}."""
+    def classes(self):
+        """Row-pass launch classes in execution order: dicts of level, jobs,
+        tallest and widest grid."""
+        n = ctypes.c_int()
+        check(lib().ps_plan_classes(self.handle, ctypes.byref(n), None, 0), "ps_plan_classes")
+        info = (ctypes.c_int * (4 * max(1, n.value)))()
+        check(lib().ps_plan_classes(self.handle, ctypes.byref(n), info, n.value), "ps_plan_classes")
+        return [dict(level=info[4 * c], jobs=info[4 * c + 1], max_h=info[4 * c + 2], max_w=info[4 * c + 3])
+                for c in range(n.value)]
+
+    def _sweep_ms(self, unary, marg, root, maps, s, reps):
+        import torch
+        with torch.cuda.stream(s):
+            self.execute(unary, marg, root, maps, stream=s)  # first use of this shape
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record(s)
+            for _ in range(reps):
+                self.execute(unary, marg, root, maps, stream=s)
+            t1.record(s)
+        t1.synchronize()
+        return t0.elapsed_time(t1) / reps
+
+    def autotune(self, unary, marg, root, maps, method_is_fp32=False, reps=2, stream=None):
+        """Choose the launch shapes of the current method by timing whole
+        sweeps on the caller's buffers (HIP events on the execution stream):
+        the BF16 methods' rows per x-pass block (candidates 2, 4, 8, 16), or
+        the FP32 row pass's threads per block (one per column, 128, 192, 256),
+        first one value for every launch class, then class by class
+        (coordinate descent: each class keeps the value that gave the fastest
+        sweep with the others fixed).  Sets the choices and returns
+        {"knob": name, "all": {candidate: ms}, "per_class": [(class, value,
+        ms)], "ms": final ms per sweep}."""
         import torch
         s = stream if stream is not None else torch.cuda.current_stream()
+        if method_is_fp32:
+            knob, name, cands = self.TUNE_FP32_ROW_THREADS, "fp32_row_threads", (0, 128, 192, 256)
+        else:
+            knob, name, cands = self.TUNE_MFMA_ROWS, "mfma_rows_per_block", (2, 4, 8, 16)
         times = {}
-        for c in candidates:
-            self.tune(self.TUNE_MFMA_ROWS, c)
-            with torch.cuda.stream(s):
-                self.execute(unary, marg, root, maps, stream=s)  # first use of this shape
-                t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                t0.record(s)
-                for _ in range(reps):
-                    self.execute(unary, marg, root, maps, stream=s)
-                t1.record(s)
-            t1.synchronize()
-            times[c] = t0.elapsed_time(t1) / reps
+        for c in cands:
+            self.tune(knob, c)
+            times[c] = self._sweep_ms(unary, marg, root, maps, s, reps)
         best = min(times, key=times.get)
-        self.tune(self.TUNE_MFMA_ROWS, best)
-        return times
+        self.tune(knob, best)
+        cur = times[best]
+        per_class = []
+        classes = self.classes()
+        if len(classes) > 1:
+            for k in range(len(classes)):
+                pick = best
+                for c in cands:
+                    if c == best:
+                        continue
+                    self.tune(knob, c, cls=k)
+                    t = self._sweep_ms(unary, marg, root, maps, s, reps)
+                    if t < cur:
+                        cur, pick = t, c
+                self.tune(knob, pick, cls=k)
+                per_class.append((k, pick, round(cur, 3)))
+        return {"knob": name, "all": {str(k): round(v, 3) for k, v in times.items()}, "per_class": per_class,
+                "ms": round(cur, 3)}
 
     def set_profiling(self, enable=True, classes=None):
         """Bracket the launches with HIP events: every kernel class, or only

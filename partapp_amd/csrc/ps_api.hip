@@ -32,7 +32,7 @@ size_t rowpass_lds_bytes(int R, int W, int kx);
 hipError_t rowpass_set_lds_limit(size_t bytes);
 hipError_t colpass_set_lds_limit(size_t bytes);
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s);
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s, int threads = 0);
 size_t colpass_lds_bytes(int ky);
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                           int R, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s);
@@ -69,7 +69,25 @@ namespace {
 struct WidthClass {
   int start = 0, count = 0, max_h = 0, max_w = 0;
   int epi_hi = kEpiUpRoot;  // highest epilogue kind of its jobs
+  int mfma_rows = 0;        // tuned rows per block of the BF16 x pass (0: the plan's)
+  int row_threads = 0;      // tuned threads per block of the FP32 row pass (0: one per column)
+  // the jobs of each instance: consecutive within the class (the level's
+  // jobs keep instance order within a class); (instance, first job relative
+  // to the level, count), by instance
+  std::vector<int> run_inst, run_first, run_count;
 };
+
+// record the instance runs of a class of a level's jobs v
+void class_runs(WidthClass &c, const std::vector<MsgJob> &v) {
+  for (int j = c.start; j < c.start + c.count; ++j) {
+    if (c.run_inst.empty() || c.run_inst.back() != v[j].inst) {
+      c.run_inst.push_back(v[j].inst);
+      c.run_first.push_back(j);
+      c.run_count.push_back(0);
+    }
+    ++c.run_count.back();
+  }
+}
 
 struct Level {
   int off = 0, count = 0, n_diag = 0, max_h = 0, max_w = 0;
@@ -82,6 +100,7 @@ struct Level {
   size_t mfma_row_lds[3] = {0, 0, 0};  // 1, 2 and 3 terms
   size_t mfma_col_lds[3] = {0, 0, 0};
   size_t mfma_col16_lds = 0;           // column pass on an fp16 row-pass output
+  WidthClass od;                       // the off-diagonal jobs [n_diag, count), for their instance runs
 };
 
 // Host bf16 of a float, round to nearest even (finite inputs).
@@ -160,6 +179,8 @@ struct ps_plan {
   std::vector<long long> keep_offsets;  // per instance, bytes
   long long keep_bytes = 0;
   unsigned *not_fallback = nullptr;     // per instance: 1 - fallback flag (the pruned pass's active set)
+  unsigned *flags_host = nullptr;       // pinned copy of the guard flags (eager execution)
+  int max_children = 0;                 // root children (MaxJobs per item)
   std::vector<Level> up, down;
   int n_roots = 0, n_maxes = 0;
   int root_children = 0;  // the same for every root job
@@ -686,8 +707,13 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         add_to(L.classes, i, *it, false);
         add_to(L.col_classes, i, *it, true);
       }
+      for (auto &c : L.classes) class_runs(c, v);
+      for (auto &c : L.col_classes) class_runs(c, v);
       L.off = (int)B.jobs.size();
       L.count = (int)v.size();
+      L.od.start = (int)(mid - v.begin());
+      L.od.count = (int)(v.end() - mid);
+      class_runs(L.od, v);
       L.mfma_ok = R == 24;
       for (MsgJob &j : v) {
         L.max_h = std::max(L.max_h, j.H);
@@ -730,6 +756,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->n_roots = (int)B.roots.size();
   pl->root_children = B.roots.empty() ? 0 : B.roots[0].K;
   pl->n_maxes = (int)B.maxes.size();
+  pl->max_children = K;
   pl->n_keys = items * 2 * P;
   pl->n_map = items * P;
   pl->scratch_elems = scratch_off;
@@ -784,7 +811,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       return set_error(PS_ERR_OUT_OF_MEMORY, "scratch (" + std::to_string(pl->scratch_elems * 4) + " bytes): " + hipGetErrorString(e));
     }
   }
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&pl->flags_host), sizeof(unsigned) * n_instances);
   if (e != hipSuccess) {
+    if (pl->scratch) (void)hipFree(pl->scratch);
     (void)hipFree(pl->tables);
     delete pl;
     return hip_error(e, "plan upload");
@@ -860,20 +889,79 @@ extern "C" int ps_plan_read_map_guard(ps_plan *plan, int *n_reevaluated, unsigne
   return PS_OK;
 }
 
-extern "C" int ps_plan_tune(ps_plan *plan, int knob, int value) {
-  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+namespace {
+// the row-pass launch classes of a plan in execution order
+std::vector<WidthClass *> tune_classes(ps_plan *plan) {
+  std::vector<WidthClass *> v;
+  for (auto *lv : {&plan->up, &plan->down})
+    for (auto &L : *lv)
+      for (auto &c : L.classes) v.push_back(&c);
+  return v;
+}
+int check_knob(int knob, int value) {
   switch (knob) {
     case PS_TUNE_MFMA_ROWS:
       if (value < 1 || value > 64) return set_error(PS_ERR_INVALID_ARG, "rows per block must be in [1, 64]");
-      if (value != plan->mfma_rows && plan->gexec) {
-        (void)hipGraphExecDestroy(plan->gexec);
-        plan->gexec = nullptr;
-      }
-      plan->mfma_rows = value;
+      return PS_OK;
+    case PS_TUNE_FP32_ROW_THREADS:
+      if (value < 0 || value > 512 || value % 64)
+        return set_error(PS_ERR_INVALID_ARG, "threads per block must be 0 or a multiple of 64 up to 512");
       return PS_OK;
     default:
       return set_error(PS_ERR_INVALID_ARG, "unknown tuning knob " + std::to_string(knob));
   }
+}
+void set_knob(WidthClass &c, int knob, int value) {
+  if (knob == PS_TUNE_MFMA_ROWS) c.mfma_rows = value;
+  else c.row_threads = value;
+}
+void drop_graph(ps_plan *plan) {
+  if (plan->gexec) {
+    (void)hipGraphExecDestroy(plan->gexec);
+    plan->gexec = nullptr;
+  }
+}
+}  // namespace
+
+extern "C" int ps_plan_tune(ps_plan *plan, int knob, int value) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  int st = check_knob(knob, value);
+  if (st != PS_OK) return st;
+  drop_graph(plan);
+  if (knob == PS_TUNE_MFMA_ROWS) plan->mfma_rows = value;
+  for (WidthClass *c : tune_classes(plan)) set_knob(*c, knob, knob == PS_TUNE_MFMA_ROWS ? 0 : value);
+  return PS_OK;
+}
+
+extern "C" int ps_plan_tune_class(ps_plan *plan, int knob, int cls, int value) {
+  if (!plan) return set_error(PS_ERR_INVALID_ARG, "plan is NULL");
+  int st = check_knob(knob, value);
+  if (st != PS_OK) return st;
+  std::vector<WidthClass *> v = tune_classes(plan);
+  if (cls < 0 || cls >= (int)v.size()) return set_error(PS_ERR_INVALID_ARG, "no launch class " + std::to_string(cls));
+  drop_graph(plan);
+  set_knob(*v[cls], knob, value);
+  return PS_OK;
+}
+
+extern "C" int ps_plan_classes(const ps_plan *plan, int *n_classes, int *info, int max_classes) {
+  if (!plan || !n_classes) return set_error(PS_ERR_INVALID_ARG, "null argument");
+  int n = 0, level = 0;
+  for (auto *lv : {&plan->up, &plan->down})
+    for (auto &L : *lv) {
+      for (auto &c : L.classes) {
+        if (info && n < max_classes) {
+          info[4 * n] = level;
+          info[4 * n + 1] = c.count;
+          info[4 * n + 2] = c.max_h;
+          info[4 * n + 3] = c.max_w;
+        }
+        ++n;
+      }
+      ++level;
+    }
+  *n_classes = n;
+  return PS_OK;
 }
 
 extern "C" int ps_plan_method_levels(const ps_plan *plan, int *levels, int *mfma_levels) {
@@ -922,6 +1010,7 @@ extern "C" void ps_plan_destroy(ps_plan *plan) {
   if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
   if (plan->scratch) (void)hipFree(plan->scratch);
   if (plan->tables) (void)hipFree(plan->tables);
+  if (plan->flags_host) (void)hipHostFree(plan->flags_host);
   delete plan;
 }
 
@@ -954,9 +1043,11 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
       const long ev2 = profile_begin(pp, 2, s);
       if (mf)
         e = launch_rowpass_mfma(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->frags, pl->R, terms,
-                                pl->mfma_rows, c.max_h, c.max_w, L.mfma_row_lds[ti], s);
+                                c.mfma_rows > 0 ? c.mfma_rows : pl->mfma_rows, c.max_h, c.max_w,
+                                L.mfma_row_lds[ti], s);
       else
-        e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s);
+        e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s,
+                           c.row_threads);
       profile_end(pp, ev2, s);
     }
     if (e == hipSuccess && n_od > 0) {  // off-diagonal messages: the FP32 row pass, then the eigenbasis stages
@@ -1007,13 +1098,84 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
   return e;
 }
 
+// The FP32 sweep of the instances whose flag is set in pl->flags_host (and,
+// on the device, in b.active): every heavy launch covers only those
+// instances' jobs, one launch per (class, instance run); the key reset runs
+// masked by b.active.
+static hipError_t sweep_subset(ps_plan *pl, const Bases &b, ps_map *map_dev, hipStream_t s, ps_plan *pp) {
+  std::vector<char> on(pl->n_inst);
+  for (int i = 0; i < pl->n_inst; ++i) on[i] = pl->flags_host[i] != 0u;
+  const int S = pl->S, P = pl->P, K = pl->max_children;
+  long ev = profile_begin(pp, 0, s);
+  hipError_t e = launch_reset(b, pl->n_keys, pl->n_map, pl->n_inst, nullptr, 0, s);
+  profile_end(pp, ev, s);
+  for (int i = 0; i < pl->n_inst && e == hipSuccess; ++i)
+    if (on[i]) {
+      ev = profile_begin(pp, 1, s);
+      e = launch_max(b, pl->maxes + (size_t)i * S * K, S * K, pl->R, pl->max_hw, s);
+      profile_end(pp, ev, s);
+    }
+  auto runs = [&](const WidthClass &c, const MsgJob *jobs, auto &&fn) {
+    for (size_t r = 0; r < c.run_inst.size() && e == hipSuccess; ++r)
+      if (on[c.run_inst[r]]) e = fn(jobs + c.run_first[r], c.run_count[r]);
+  };
+  auto level = [&](const Level &L) {
+    const MsgJob *jobs = pl->jobs + L.off;
+    for (const WidthClass &c : L.classes) {
+      const long ev2 = profile_begin(pp, 2, s);
+      runs(c, jobs, [&](const MsgJob *j, int n) {
+        return launch_rowpass(b, j, n, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s, c.row_threads);
+      });
+      profile_end(pp, ev2, s);
+    }
+    if (L.od.count > 0) {
+      const long ev2 = profile_begin(pp, 2, s);
+      runs(L.od, jobs, [&](const MsgJob *j, int n) {
+        return launch_rowpass(b, j, n, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
+      });
+      profile_end(pp, ev2, s);
+    }
+    for (const WidthClass &c : L.col_classes) {
+      const long ev2 = profile_begin(pp, 3, s);
+      runs(c, jobs, [&](const MsgJob *j, int n) {
+        return launch_colpass(b, j, n, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s);
+      });
+      profile_end(pp, ev2, s);
+    }
+    if (L.od.count > 0) {
+      const long ev2 = profile_begin(pp, 3, s);
+      runs(L.od, jobs, [&](const MsgJob *j, int n) {
+        return launch_offdiag(b, j, n, pl->taps, pl->shifts, pl->R, L.od_cells, L.od_hw, s);
+      });
+      profile_end(pp, ev2, s);
+    }
+  };
+  for (const Level &L : pl->up)
+    if (e == hipSuccess) level(L);
+  for (int i = 0; i < pl->n_inst && e == hipSuccess; ++i)
+    if (on[i]) {
+      ev = profile_begin(pp, 4, s);
+      e = launch_root(b, pl->roots + (size_t)i * S, S, pl->root_children, pl->R, pl->max_hw, s);
+      profile_end(pp, ev, s);
+    }
+  for (const Level &L : pl->down)
+    if (e == hipSuccess) level(L);
+  for (int i = 0; i < pl->n_inst && e == hipSuccess && map_dev; ++i)
+    if (on[i]) {
+      ev = profile_begin(pp, 5, s);
+      e = launch_map_decode(b, pl->maps + (size_t)i * S * P, S * P, map_dev, s);
+      profile_end(pp, ev, s);
+    }
+  return e;
+}
+
 // The plan's method; with the MAP guard on and a BF16 method, every
 // instance whose MAP certificate fails (some part's best log marginal beats
 // its second best by at most guard_tau) is swept again in FP32, on the
 // device, in the same stream order.
 static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, float *root_posterior, ps_map *map_dev,
                    hipStream_t s, bool profile, const unsigned char *keep = nullptr,
-                   const unsigned *fallback = nullptr) {
+                   const unsigned *fallback = nullptr, bool capturing = false) {
   if (keep) {
     // a pruned execution: the instances without the fallback flag run on
     // their kept tiles, then the flagged ones run the whole sweep
@@ -1058,11 +1220,21 @@ static int enqueue(ps_plan *pl, const float *log_unary, float *log_marginal, flo
   hipError_t e = sweep(pl, b, pl->terms, pl->col_terms, map_dev, s, pp, pl->flags);
   if (e == hipSuccess && guard) {
     e = launch_certify(pl->top2, pl->n_map, pl->n_inst, pl->guard_tau, pl->flags, s);
-    if (e == hipSuccess) {
-      Bases bf = b;
-      bf.top2_keys = nullptr;
-      bf.active = pl->flags;
+    Bases bf = b;
+    bf.top2_keys = nullptr;
+    bf.active = pl->flags;
+    if (e == hipSuccess && capturing) {
+      // under capture the flags stay on the device: every FP32 launch covers
+      // the whole batch and the blocks of unflagged instances exit at once
       e = sweep(pl, bf, 0, 0, map_dev, s, pp, nullptr);
+    } else if (e == hipSuccess) {
+      // eager: read the flags, then launch the FP32 passes over the flagged
+      // instances' jobs only
+      e = hipMemcpyAsync(pl->flags_host, pl->flags, sizeof(unsigned) * pl->n_inst, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      bool any = false;
+      for (int i = 0; i < pl->n_inst && e == hipSuccess; ++i) any = any || pl->flags_host[i] != 0u;
+      if (e == hipSuccess && any) e = sweep_subset(pl, bf, map_dev, s, pp);
     }
   }
   if (e != hipSuccess) return hip_error(e, "kernel launch");
@@ -1120,7 +1292,7 @@ extern "C" int ps_plan_execute_graph(ps_plan *plan, const float *log_unary, floa
     }
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_error(e, "hipStreamBeginCapture");
-    st = enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, s, false);
+    st = enqueue(plan, log_unary, log_marginal, root_posterior, map_dev, s, false, nullptr, nullptr, true);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     if (st != PS_OK) {

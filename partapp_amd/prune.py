@@ -137,6 +137,8 @@ class Pruner:
     def __init__(self, plan, device, net=None, part_budget=PART_BUDGET, fallback_mass=FALLBACK_MASS):
         torch = _torch()
         self.plan = plan
+        # let the convolution library time its algorithms for these shapes once
+        torch.backends.cudnn.benchmark = True
         self.net = net if net is not None else load_net(device, P=plan.P, R=plan.R)
         self.part_budget = part_budget
         self.fallback_mass = fallback_mass

@@ -101,3 +101,45 @@ def test_default_rotation_grid_48(gpu):
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
         assert np.array_equal(got[2], ref[2])
     plan.close()
+
+
+def test_launch_shape_knobs_do_not_change_results(gpu):
+    # the FP32 row pass at 64..512 threads per block (a thread then covers
+    # several columns) and per-class settings of both knobs: bit for bit the
+    # default launch shapes
+    import torch
+    from partapp_amd._lib import PsError
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = [synth_joints(47), synth_joints(48)]
+    model = person_model(ep, joints[0])
+    sizes = [(72, 136), (64, 200)]
+    us = [synth_unary(49 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(sizes)]
+    plan = Plan(model, sizes, joints)
+    classes = plan.classes()
+    say("launch classes:", classes)
+    assert len(classes) >= 4 and all(c["jobs"] > 0 for c in classes)
+    for method, knob, values in (("fp32", plan.TUNE_FP32_ROW_THREADS, (64, 128, 256, 512)),
+                                 ("bf16mix", plan.TUNE_MFMA_ROWS, (1, 3, 16))):
+        plan.set_method(method)
+        plan.tune(knob, 0 if method == "fp32" else 4)
+        un, marg, root, maps = upload(plan, us, gpu)
+        plan.execute(un, marg, root, maps)
+        ref = [t.clone() for t in (marg, root, maps)]
+        for v in values:
+            plan.tune(knob, v)
+            marg.fill_(float("nan"))
+            plan.execute(un, marg, root, maps)
+            torch.cuda.synchronize()
+            assert torch.equal(marg, ref[0]) and torch.equal(root, ref[1]) and torch.equal(maps, ref[2]), (method, v)
+        plan.tune(knob, 0 if method == "fp32" else 4)
+        plan.tune(knob, values[0], cls=1)
+        plan.tune(knob, values[-1], cls=2)
+        marg.fill_(float("nan"))
+        plan.execute(un, marg, root, maps)
+        torch.cuda.synchronize()
+        assert torch.equal(marg, ref[0]), method
+    with pytest.raises(PsError):
+        plan.tune(plan.TUNE_FP32_ROW_THREADS, 100)
+    with pytest.raises(PsError):
+        plan.tune(plan.TUNE_MFMA_ROWS, 4, cls=len(classes))
+    plan.close()

@@ -292,14 +292,15 @@ class Plan:
         t1.synchronize()
         return t0.elapsed_time(t1) / reps
 
-    def autotune(self, unary, marg, root, maps, method_is_fp32=False, reps=2, stream=None):
+    def autotune(self, unary, marg, root, maps, method_is_fp32=False, reps=2, stream=None, max_classes=8):
         """Choose the launch shapes of the current method by timing whole
         sweeps on the caller's buffers (HIP events on the execution stream):
         the BF16 methods' rows per x-pass block (candidates 2, 4, 8, 16), or
         the FP32 row pass's threads per block (one per column, 128, 192, 256),
         first one value for every launch class, then class by class
         (coordinate descent: each class keeps the value that gave the fastest
-        sweep with the others fixed).  Sets the choices and returns
+        sweep with the others fixed; skipped above max_classes classes, where
+        it would take more sweeps than it can save).  Sets the choices and returns
         {"knob": name, "all": {candidate: ms}, "per_class": [(class, value,
         ms)], "ms": final ms per sweep}."""
         import torch
@@ -317,7 +318,7 @@ class Plan:
         cur = times[best]
         per_class = []
         classes = self.classes()
-        if len(classes) > 1:
+        if 1 < len(classes) <= max_classes:
             for k in range(len(classes)):
                 pick = best
                 for c in cands:

@@ -181,6 +181,15 @@ struct ps_plan {
   unsigned *not_fallback = nullptr;     // per instance: 1 - fallback flag (the pruned pass's active set)
   unsigned *flags_host = nullptr;       // pinned copy of the guard flags (eager execution)
   int max_children = 0;                 // root children (MaxJobs per item)
+  // the flagged instances' jobs gathered for the guard's FP32 re-sweep:
+  // host copies of the tables, a pinned staging block and its device twin
+  std::vector<MsgJob> h_jobs;
+  std::vector<RootJob> h_roots;
+  std::vector<MaxJob> h_maxes;
+  std::vector<MapJob> h_maps;
+  char *cstage = nullptr;
+  char *ctables = nullptr;
+  size_t cbytes = 0;
   std::vector<Level> up, down;
   int n_roots = 0, n_maxes = 0;
   int root_children = 0;  // the same for every root job
@@ -812,12 +821,25 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     }
   }
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&pl->flags_host), sizeof(unsigned) * n_instances);
+  // room for every job twice (row-pass and column-pass classes list the same
+  // jobs) and every root, max and map job, each part 256-byte aligned
+  pl->cbytes = align(2 * B.jobs.size() * sizeof(MsgJob)) + align(B.roots.size() * sizeof(RootJob)) +
+               align(B.maxes.size() * sizeof(MaxJob)) + align(B.maps.size() * sizeof(MapJob));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&pl->cstage), pl->cbytes);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&pl->ctables), pl->cbytes);
   if (e != hipSuccess) {
+    if (pl->ctables) (void)hipFree(pl->ctables);
+    if (pl->cstage) (void)hipHostFree(pl->cstage);
+    if (pl->flags_host) (void)hipHostFree(pl->flags_host);
     if (pl->scratch) (void)hipFree(pl->scratch);
     (void)hipFree(pl->tables);
     delete pl;
     return hip_error(e, "plan upload");
   }
+  pl->h_jobs = B.jobs;
+  pl->h_roots = B.roots;
+  pl->h_maxes = B.maxes;
+  pl->h_maps = B.maps;
   size_t max_lds = 0, max_col_lds = 0;
   for (auto *lv : {&pl->up, &pl->down})
     for (auto &L : *lv) {
@@ -1011,6 +1033,8 @@ extern "C" void ps_plan_destroy(ps_plan *plan) {
   if (plan->scratch) (void)hipFree(plan->scratch);
   if (plan->tables) (void)hipFree(plan->tables);
   if (plan->flags_host) (void)hipHostFree(plan->flags_host);
+  if (plan->cstage) (void)hipHostFree(plan->cstage);
+  if (plan->ctables) (void)hipFree(plan->ctables);
   delete plan;
 }
 
@@ -1099,73 +1123,112 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
 }
 
 // The FP32 sweep of the instances whose flag is set in pl->flags_host (and,
-// on the device, in b.active): every heavy launch covers only those
-// instances' jobs, one launch per (class, instance run); the key reset runs
-// masked by b.active.
+// on the device, in b.active): their jobs are gathered into compact tables
+// (one segment per launch class, uploaded from pinned memory in stream
+// order), so every launch covers exactly the flagged instances' jobs; the
+// key reset runs masked by b.active.
 static hipError_t sweep_subset(ps_plan *pl, const Bases &b, ps_map *map_dev, hipStream_t s, ps_plan *pp) {
   std::vector<char> on(pl->n_inst);
   for (int i = 0; i < pl->n_inst; ++i) on[i] = pl->flags_host[i] != 0u;
-  const int S = pl->S, P = pl->P, K = pl->max_children;
+  struct Seg {
+    size_t start = 0;
+    int count = 0;
+  };
+  std::vector<MsgJob> cj;
+  auto gather = [&](const Level &L, const WidthClass &c) {
+    Seg g;
+    g.start = cj.size();
+    for (size_t r = 0; r < c.run_inst.size(); ++r)
+      if (on[c.run_inst[r]])
+        for (int j = 0; j < c.run_count[r]; ++j) cj.push_back(pl->h_jobs[L.off + c.run_first[r] + j]);
+    g.count = (int)(cj.size() - g.start);
+    return g;
+  };
+  struct LevelSegs {
+    std::vector<Seg> row, col;
+    Seg od_row, od_col;
+  };
+  auto segs_of = [&](const std::vector<Level> &lv) {
+    std::vector<LevelSegs> out;
+    for (const Level &L : lv) {
+      LevelSegs ls;
+      for (const WidthClass &c : L.classes) ls.row.push_back(gather(L, c));
+      for (const WidthClass &c : L.col_classes) ls.col.push_back(gather(L, c));
+      ls.od_row = gather(L, L.od);
+      ls.od_col = gather(L, L.od);
+      out.push_back(ls);
+    }
+    return out;
+  };
+  const std::vector<LevelSegs> up = segs_of(pl->up), down = segs_of(pl->down);
+  std::vector<RootJob> cr;
+  std::vector<MaxJob> cm;
+  std::vector<MapJob> cp;
+  for (const RootJob &j : pl->h_roots)
+    if (on[j.inst]) cr.push_back(j);
+  for (const MaxJob &j : pl->h_maxes)
+    if (on[j.inst]) cm.push_back(j);
+  for (const MapJob &j : pl->h_maps)
+    if (on[j.inst]) cp.push_back(j);
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t o_r = align(cj.size() * sizeof(MsgJob)), o_m = o_r + align(cr.size() * sizeof(RootJob));
+  const size_t o_p = o_m + align(cm.size() * sizeof(MaxJob)), n_bytes = o_p + cp.size() * sizeof(MapJob);
+  if (n_bytes > pl->cbytes) return hipErrorInvalidValue;
+  std::memcpy(pl->cstage, cj.data(), cj.size() * sizeof(MsgJob));
+  std::memcpy(pl->cstage + o_r, cr.data(), cr.size() * sizeof(RootJob));
+  std::memcpy(pl->cstage + o_m, cm.data(), cm.size() * sizeof(MaxJob));
+  std::memcpy(pl->cstage + o_p, cp.data(), cp.size() * sizeof(MapJob));
+  hipError_t e = hipMemcpyAsync(pl->ctables, pl->cstage, n_bytes, hipMemcpyHostToDevice, s);
+  const MsgJob *jobs = reinterpret_cast<const MsgJob *>(pl->ctables);
   long ev = profile_begin(pp, 0, s);
-  hipError_t e = launch_reset(b, pl->n_keys, pl->n_map, pl->n_inst, nullptr, 0, s);
+  if (e == hipSuccess) e = launch_reset(b, pl->n_keys, pl->n_map, pl->n_inst, nullptr, 0, s);
   profile_end(pp, ev, s);
-  for (int i = 0; i < pl->n_inst && e == hipSuccess; ++i)
-    if (on[i]) {
-      ev = profile_begin(pp, 1, s);
-      e = launch_max(b, pl->maxes + (size_t)i * S * K, S * K, pl->R, pl->max_hw, s);
-      profile_end(pp, ev, s);
-    }
-  auto runs = [&](const WidthClass &c, const MsgJob *jobs, auto &&fn) {
-    for (size_t r = 0; r < c.run_inst.size() && e == hipSuccess; ++r)
-      if (on[c.run_inst[r]]) e = fn(jobs + c.run_first[r], c.run_count[r]);
-  };
-  auto level = [&](const Level &L) {
-    const MsgJob *jobs = pl->jobs + L.off;
-    for (const WidthClass &c : L.classes) {
+  if (e == hipSuccess) {
+    ev = profile_begin(pp, 1, s);
+    e = launch_max(b, reinterpret_cast<const MaxJob *>(pl->ctables + o_m), (int)cm.size(), pl->R, pl->max_hw, s);
+    profile_end(pp, ev, s);
+  }
+  auto level = [&](const Level &L, const LevelSegs &ls) {
+    for (size_t k = 0; k < L.classes.size() && e == hipSuccess; ++k) {
+      const WidthClass &c = L.classes[k];
       const long ev2 = profile_begin(pp, 2, s);
-      runs(c, jobs, [&](const MsgJob *j, int n) {
-        return launch_rowpass(b, j, n, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s, c.row_threads);
-      });
+      e = launch_rowpass(b, jobs + ls.row[k].start, ls.row[k].count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w,
+                         L.lds, s, c.row_threads);
       profile_end(pp, ev2, s);
     }
-    if (L.od.count > 0) {
+    if (e == hipSuccess && ls.od_row.count > 0) {
       const long ev2 = profile_begin(pp, 2, s);
-      runs(L.od, jobs, [&](const MsgJob *j, int n) {
-        return launch_rowpass(b, j, n, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
-      });
+      e = launch_rowpass(b, jobs + ls.od_row.start, ls.od_row.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w,
+                         L.lds, s);
       profile_end(pp, ev2, s);
     }
-    for (const WidthClass &c : L.col_classes) {
+    for (size_t k = 0; k < L.col_classes.size() && e == hipSuccess; ++k) {
+      const WidthClass &c = L.col_classes[k];
       const long ev2 = profile_begin(pp, 3, s);
-      runs(c, jobs, [&](const MsgJob *j, int n) {
-        return launch_colpass(b, j, n, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s);
-      });
+      e = launch_colpass(b, jobs + ls.col[k].start, ls.col[k].count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w,
+                         L.col_lds, s);
       profile_end(pp, ev2, s);
     }
-    if (L.od.count > 0) {
+    if (e == hipSuccess && ls.od_col.count > 0) {
       const long ev2 = profile_begin(pp, 3, s);
-      runs(L.od, jobs, [&](const MsgJob *j, int n) {
-        return launch_offdiag(b, j, n, pl->taps, pl->shifts, pl->R, L.od_cells, L.od_hw, s);
-      });
+      e = launch_offdiag(b, jobs + ls.od_col.start, ls.od_col.count, pl->taps, pl->shifts, pl->R, L.od_cells,
+                         L.od_hw, s);
       profile_end(pp, ev2, s);
     }
   };
-  for (const Level &L : pl->up)
-    if (e == hipSuccess) level(L);
-  for (int i = 0; i < pl->n_inst && e == hipSuccess; ++i)
-    if (on[i]) {
-      ev = profile_begin(pp, 4, s);
-      e = launch_root(b, pl->roots + (size_t)i * S, S, pl->root_children, pl->R, pl->max_hw, s);
-      profile_end(pp, ev, s);
-    }
-  for (const Level &L : pl->down)
-    if (e == hipSuccess) level(L);
-  for (int i = 0; i < pl->n_inst && e == hipSuccess && map_dev; ++i)
-    if (on[i]) {
-      ev = profile_begin(pp, 5, s);
-      e = launch_map_decode(b, pl->maps + (size_t)i * S * P, S * P, map_dev, s);
-      profile_end(pp, ev, s);
-    }
+  for (size_t l = 0; l < pl->up.size() && e == hipSuccess; ++l) level(pl->up[l], up[l]);
+  if (e == hipSuccess) {
+    ev = profile_begin(pp, 4, s);
+    e = launch_root(b, reinterpret_cast<const RootJob *>(pl->ctables + o_r), (int)cr.size(), pl->root_children,
+                    pl->R, pl->max_hw, s);
+    profile_end(pp, ev, s);
+  }
+  for (size_t l = 0; l < pl->down.size() && e == hipSuccess; ++l) level(pl->down[l], down[l]);
+  if (e == hipSuccess && map_dev) {
+    ev = profile_begin(pp, 5, s);
+    e = launch_map_decode(b, reinterpret_cast<const MapJob *>(pl->ctables + o_p), (int)cp.size(), map_dev, s);
+    profile_end(pp, ev, s);
+  }
   return e;
 }
 

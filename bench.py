@@ -447,10 +447,20 @@ def main():
     # the same method with the MAP guard: instances whose MAP certificate
     # fails are swept again in FP32 within the step
     guarded = None
+    early = {}  # checks made before a later leg reuses the chosen leg's buffers
+    k_check = n if args.check is None else min(args.check, n)
     tau = Plan.MAP_GUARD_TAU.get(args.method, 0.0)
     if tau > 0 and not args.no_guard:
-        margg = torch.empty_like(unary)
-        mapsg = torch.zeros_like(maps)
+        if args.config == 1:
+            margg = torch.empty_like(unary)
+            mapsg = torch.zeros_like(maps)
+        else:
+            # no room for another 75 GB output: check the unguarded leg now,
+            # then let the guarded leg overwrite its buffers
+            if k_check > 0:
+                early = check_instances(plan, model, unary, {args.method: (marg, maps)}, ep, joints,
+                                        list(range(k_check)))
+            margg, mapsg = marg, maps
         msg_, _ = time_method(plan, args.method, unary, margg, root, mapsg, stream, args, guard=tau)
         n_re, _ = plan.read_map_guard(stream)
         plan.set_map_guard(0.0)
@@ -507,11 +517,13 @@ def main():
                 args.method, msp / args.steps, pr.kept_fraction(), int(fb.sum()), n), file=sys.stderr, flush=True)
 
     err = None
-    k = n if args.check is None else min(args.check, n)
+    k = k_check
     if k > 0:
-        legs[args.method] = (marg, maps)
+        if args.method not in early:
+            legs[args.method] = (marg, maps)
         tc = time.perf_counter()
         chk = check_instances(plan, model, unary, legs, ep, joints, list(range(k)), pruned=prune_info)
+        chk.update(early)
         err = chk[args.method]
         if fp32 is not None and "fp32" in chk:
             fp32["parity"] = chk["fp32"]

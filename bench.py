@@ -52,6 +52,8 @@ def parse():
                          "size (H, W in 160..480, multiples of 8) x 24 rotations x 3 scales")
     ap.add_argument("--instances", type=int, default=None, help="default 32 (config 1) / 256 (config 2)")
     ap.add_argument("--size", type=int, default=300, help="H = W of config 1")
+    ap.add_argument("--rotations", type=int, default=24,
+                    help="rotation steps over 360 degrees (24 in the metric; 48 is partapp's default)")
     ap.add_argument("--method", default="bf16mix", choices=["bf16mix", "bf16x3", "bf16x2", "bf16", "fp32"],
                     help="the chosen method (value); plain FP32 is timed beside it")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison")
@@ -364,12 +366,16 @@ def main():
     import numpy as np
     import torch
 
-    from partapp_amd.api import Plan, config_exp_param, person_model, synth_joints, synth_unary, uneven_sizes
+    from partapp_amd.api import (ExpParam, Plan, config_exp_param, person_model, synth_joints, synth_unary,
+                                 uneven_sizes)
 
     from partapp_amd.api import synth_unary_dev
 
     dev = torch.device("cuda:0")
     ep = config_exp_param(args.config)
+    if args.rotations != 24:
+        ep = ExpParam(ep.min_part_rotation, ep.max_part_rotation, args.rotations, ep.min_object_scale,
+                      ep.max_object_scale, ep.num_scale_steps, ep.strip_border_detections)
     R, S = ep.num_rotation_steps, ep.num_scale_steps
     if args.config == 1:
         n = args.instances or 32
@@ -547,8 +553,8 @@ def main():
         launches_per_step = n_dom / args.steps
         avg_ms = t_dom / n_dom
         achieved = bytes_per_step / launches_per_step / (avg_ms / 1e3) / 1e9
-        traffic, tsrc = (measured_traffic(args.method, dom) if args.config == 1 and n == 32 and args.size == 300
-                         else (None, None))
+        traffic, tsrc = (measured_traffic(args.method, dom)
+                         if args.config == 1 and n == 32 and args.size == 300 and R == 24 else (None, None))
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "algorithmic_bytes_per_launch": round(bytes_per_step / launches_per_step),

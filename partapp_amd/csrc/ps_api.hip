@@ -307,7 +307,11 @@ struct Builder {
   // 16 kk + 8 (t / 4) + 4 h + t % 4, so lane l, step kk, element t of the
   // table is F[l % 32][that slice] with F[ro][s] the weight of source slice s
   // in output slice ro (circular taps, objectdetect_findrot.cpp:377-420);
-  // zero for slices >= R.  The hi block, then the lo block.
+  // zero for slices >= R.  The hi block, then the lo block.  With 48 slices
+  // there are two output tiles (ro = 32 mt + l % 32) and three k-steps: step
+  // ks reads the accumulator of slice tile ks / 2, so its B position holds
+  // slice 32 (ks / 2) + 16 (ks % 2) + 8 (t / 4) + 4 h + t % 4; the table is
+  // [mt][ks] per block.
   int add_rot_frags(const MessagePlan &mp, int R) {
     const int off = (int)(frags.size() / 8);
     std::vector<float> F((size_t)R * R, 0.0f);
@@ -320,16 +324,18 @@ struct Builder {
       for (int ro = 0; ro < R; ++ro) F[(size_t)ro * R + ro] = 1.0f;
     }
     std::vector<uint16_t> lo;
-    for (int kk = 0; kk < 2; ++kk)
-      for (int l = 0; l < 64; ++l)
-        for (int t = 0; t < 8; ++t) {
-          const int ro = l % 32, h = l / 32;
-          const int sl = 16 * kk + 8 * (t / 4) + 4 * h + t % 4;
-          const float v = (ro < R && sl < R) ? F[(size_t)ro * R + sl] : 0.0f;
-          const uint16_t hi = bf16_rne(v);
-          frags.push_back(hi);
-          lo.push_back(bf16_rne(v - bf16_float(hi)));
-        }
+    const int MT = (R + 31) / 32, KS = (R + 15) / 16;
+    for (int mt = 0; mt < MT; ++mt)
+      for (int ks = 0; ks < KS; ++ks)
+        for (int l = 0; l < 64; ++l)
+          for (int t = 0; t < 8; ++t) {
+            const int ro = 32 * mt + l % 32, h = l / 32;
+            const int sl = 32 * (ks / 2) + 16 * (ks % 2) + 8 * (t / 4) + 4 * h + t % 4;
+            const float v = (ro < R && sl < R) ? F[(size_t)ro * R + sl] : 0.0f;
+            const uint16_t hi = bf16_rne(v);
+            frags.push_back(hi);
+            lo.push_back(bf16_rne(v - bf16_float(hi)));
+          }
     frags.insert(frags.end(), lo.begin(), lo.end());
     return off;
   }
@@ -378,7 +384,7 @@ struct Builder {
         J.frag_x = add_frags(mp.fx, J.kk_x);
         J.frag_y = add_frags(mp.fy, J.kk_y);
         J.frag_y16 = add_taps16(mp.fy, J.kk_y);
-        J.frag_r = R <= 24 ? add_rot_frags(mp, R) : 0;
+        J.frag_r = (R == 24 || R == 48) ? add_rot_frags(mp, R) : 0;
         // the row-pass output is at most (sum of x taps) (largest row sum of
         // the rotation filter) times max exp(b - M) = 1; its fp16 copy holds
         // it scaled by 2^t16_shift, that bound (with margin) near 2^14
@@ -723,7 +729,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       L.od.start = (int)(mid - v.begin());
       L.od.count = (int)(v.end() - mid);
       class_runs(L.od, v);
-      L.mfma_ok = R == 24;
+      L.mfma_ok = R == 24 || R == 48;
       for (MsgJob &j : v) {
         L.max_h = std::max(L.max_h, j.H);
         L.max_w = std::max(L.max_w, j.W);

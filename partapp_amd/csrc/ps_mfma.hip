@@ -123,12 +123,26 @@ __host__ __device__ __forceinline__ int mfma_ring_pitch(int KK) { return mfma_ri
 // a job's width (the block is sized for the widest job of the launch) only
 // gather and take part in the barriers.
 //
-// At most 72 VGPRs (7 waves per SIMD): five 320-thread blocks per CU for a
-// 300-wide grid instead of four, at the cost of two spilled VGPRs.
+// R = 24: one 32-row slice tile (slices 24..31 padding), a 2-step rotation
+// GEMM.  R = 48: two slice tiles for the x filter, the rotation GEMM over 3
+// k-steps (slices 0..31 from the first tile's accumulator, 32..47 from the
+// second's) into two output tiles.
+//
+// R = 24 at most 72 VGPRs (7 waves per SIMD): five 320-thread blocks per CU
+// for a 300-wide grid instead of four, at the cost of two spilled VGPRs.
+template <int R>
+struct RowTiles {
+  static constexpr int MT = (R + 31) / 32;  // 32-slice tiles of the x filter and of the rotation output
+  static constexpr int KS = (R + 15) / 16;  // k-steps of the rotation GEMM
+  static constexpr int NRES = R == 24 ? 12 : 24;  // outputs per lane and column tile
+};
+
 template <int R, int TERMS, bool T16>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k_rowpass_mfma(
-    Bases b, const MsgJob *__restrict__ jobs, const int *__restrict__ shifts, const uint4 *__restrict__ frags, int rows) {
-  static_assert(R <= 24, "the rotation GEMM keeps output slices 0..23");
+__device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restrict__ jobs,
+                                                  const int *__restrict__ shifts, const uint4 *__restrict__ frags,
+                                                  int rows) {
+  static_assert(R == 24 || R == 48, "the MFMA row pass takes 24 or 48 rotation slices");
+  constexpr int MT = RowTiles<R>::MT, KS = RowTiles<R>::KS, NRES = RowTiles<R>::NRES;
   extern __shared__ uint4 lds[];
   unsigned bx, by;
   xcd_block(bx, by);
@@ -143,14 +157,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
   constexpr int nr = 2;                   // rotation tap parts (always hi + lo)
   const int pitch = mfma_row_pitch(W, kx, KK);
   uint4 *fr = lds;                 // [nf][KK][64] Toeplitz fragments
-  uint4 *frr = fr + nf * KK * 64;  // [nr][2][64] rotation fragments
-  unsigned short *Ph = reinterpret_cast<unsigned short *>(frr + nr * 2 * 64);
+  uint4 *frr = fr + nf * KK * 64;  // [nr][MT][KS][64] rotation fragments
+  unsigned short *Ph = reinterpret_cast<unsigned short *>(frr + nr * MT * KS * 64);
   unsigned short *Pl = Ph + R * pitch;
 
   const uint4 *gf = frags + J.frag_x;
   for (int i = threadIdx.x; i < nf * KK * 64; i += blockDim.x) fr[i] = gf[i];
   const uint4 *gr = frags + J.frag_r;
-  for (int i = threadIdx.x; i < nr * 2 * 64; i += blockDim.x) frr[i] = gr[i];
+  for (int i = threadIdx.x; i < nr * MT * KS * 64; i += blockDim.x) frr[i] = gr[i];
   // the halo columns stay zero for every row of the block
   const int per_row = pitch - W;
   for (int i = threadIdx.x; i < R * per_row; i += blockDim.x) {
@@ -272,8 +286,49 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
   char *Tbase = reinterpret_cast<char *>(mptr(b, J.tmp));
 
   // x filter and rotation of one 32-column tile; res[i] is output slice
-  // 8 (i / 4) + 4 h + i % 4 (< 24) of column 32 t + lane % 32
-  auto tile = [&](int t, float (&res)[12]) {
+  // 8 (i / 4) + 4 h + i % 4 (< 24) of column 32 t + lane % 32 (R = 48: res[i]
+  // for i < 16, then slice 32 + 8 (j / 4) + 4 h + j % 4 in res[16 + j], j < 8)
+  auto tile = [&](int t, float (&res)[NRES]) {
+    if constexpr (R == 48) {
+      f32x16_t c[2];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        const int mrow = min(32 * mt + (lane & 31), R - 1);
+        const unsigned short *ah = Ph + mrow * pitch + (t << 5) + kq;
+        const unsigned short *al = Pl + mrow * pitch + (t << 5) + kq;
+        f32x16_t cc = {};
+        for (int kk = 0; kk < KK; ++kk) {
+          const bf16x8_t a_hi = ld_frag(ah + 16 * kk);
+          const bf16x8_t b_hi = ld_frag(fr + kk * 64 + lane);
+          cc = mfma(a_hi, b_hi, cc);
+          if (TERMS >= 2) cc = mfma(ld_frag(al + 16 * kk), b_hi, cc);
+          if (TERMS == 3) cc = mfma(a_hi, ld_frag(fr + (KK + kk) * 64 + lane), cc);
+        }
+        c[mt] = cc;
+      }
+      // k-step ks takes accumulator tile ks / 2, registers 8 (ks % 2) ..
+      f32x16_t d[2] = {};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        float w[8];
+#pragma unroll
+        for (int t8 = 0; t8 < 8; ++t8) w[t8] = c[ks >> 1][8 * (ks & 1) + t8];
+        bf16x8_t bh, bl;
+        split8(w, bh, bl);
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+          const bf16x8_t f = ld_frag(frr + 64 * (mt * KS + ks) + lane);
+          d[mt] = mfma(f, bh, d[mt]);
+          d[mt] = mfma(f, bl, d[mt]);
+          d[mt] = mfma(ld_frag(frr + 64 * (MT * KS) + 64 * (mt * KS + ks) + lane), bh, d[mt]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) res[i] = d[0][i];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) res[16 + i] = d[1][i];
+      return;
+    }
     const unsigned short *ah = Ph + m * pitch + (t << 5) + kq;
     const unsigned short *al = Pl + m * pitch + (t << 5) + kq;
     f32x16_t c = {};
@@ -302,14 +357,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
 #pragma unroll
     for (int i = 0; i < 12; ++i) res[i] = d[i];
   };
-  auto store_tile = [&](unsigned ob, char *Trow, const float (&res)[12]) {
+  auto store_tile = [&](unsigned ob, char *Trow, const float (&res)[NRES]) {
 #pragma unroll
-    for (int i = 0; i < 12; ++i) {
-      char *p = Trow + ob + (unsigned)(8 * (i >> 2) + (i & 3)) * slice_b;
+    for (int k = 0; k < NRES; ++k) {
+      const int i = k < 16 ? k : k - 16;
+      char *p = Trow + ob + (unsigned)((k < 16 ? 0 : 32) + 8 * (i >> 2) + (i & 3)) * slice_b;
       if (T16)
-        *reinterpret_cast<_Float16 *>(p) = (_Float16)(res[i] * t_scale);
+        *reinterpret_cast<_Float16 *>(p) = (_Float16)(res[k] * t_scale);
       else
-        *reinterpret_cast<float *>(p) = res[i];
+        *reinterpret_cast<float *>(p) = res[k];
     }
   };
 
@@ -325,7 +381,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
     gather(min(y + 1, y1 - 1), v);
     __syncthreads();  // row y is in LDS
     char *Trow = Tbase + (long long)y * tp * eb;
-    float res[12];
+    float res[NRES];
     if (2 * wave < ntile) {  // waves past the grid's width (a narrower job of the launch) idle
       tile(t0, res);
       store_tile(ob0, Trow, res);
@@ -337,6 +393,19 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k
     }
     __syncthreads();  // P is rewritten by the next row
   }
+}
+
+template <int R, int TERMS, bool T16>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k_rowpass_mfma(
+    Bases b, const MsgJob *__restrict__ jobs, const int *__restrict__ shifts, const uint4 *__restrict__ frags, int rows) {
+  rowpass_mfma_body<R, TERMS, T16>(b, jobs, shifts, frags, rows);
+}
+// 48 slices hold twice the accumulators: no occupancy target
+template <int TERMS, bool T16>
+__global__ __launch_bounds__(512) void k_rowpass_mfma48(Bases b, const MsgJob *__restrict__ jobs,
+                                                        const int *__restrict__ shifts,
+                                                        const uint4 *__restrict__ frags, int rows) {
+  rowpass_mfma_body<48, TERMS, T16>(b, jobs, shifts, frags, rows);
 }
 
 // ---------------------------------------------------- column pass (y filter)
@@ -800,7 +869,8 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
 
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
-  return 16 * ((size_t)nf * KK + 4) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
+  const int rot = 2 * ((R + 31) / 32) * ((R + 15) / 16);  // hi / lo rotation fragments per lane
+  return 16 * ((size_t)nf * KK + rot) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
 }
 size_t colpass_mfma_lds_bytes(int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
@@ -818,6 +888,9 @@ hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
   hipError_t e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma48<1, true>), row_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma48<2, true>), row_bytes);
+  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma48<3, false>), row_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, false>), col_bytes);
   if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, true>), col_bytes);
@@ -837,9 +910,18 @@ hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, c
                                size_t lds_bytes, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
   // one lane per column: the plan only routes grids of width <= 512 here
-  if (R != 24 || terms < 1 || terms > 3 || max_w > 512 || rows < 1) return hipErrorInvalidValue;
+  if ((R != 24 && R != 48) || terms < 1 || terms > 3 || max_w > 512 || rows < 1) return hipErrorInvalidValue;
   dim3 grid((max_h + rows - 1) / rows, n_jobs);
   const int nt = std::min(512, std::max(64, (max_w + 63) / 64 * 64));
+  if (R == 48) {
+    if (terms == 3)
+      hipLaunchKernelGGL((k_rowpass_mfma48<3, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    else if (terms == 2)
+      hipLaunchKernelGGL((k_rowpass_mfma48<2, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    else
+      hipLaunchKernelGGL((k_rowpass_mfma48<1, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    return hipGetLastError();
+  }
   if (terms == 3)
     hipLaunchKernelGGL((k_rowpass_mfma<24, 3, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   else if (terms == 2)

@@ -322,3 +322,23 @@ def test_rows_per_block_knob_does_not_change_results(gpu):
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("method", ["bf16mix", "bf16x3"])
+def test_default_rotation_grid_48_on_the_matrix_cores(gpu, method):
+    # the reference's default 48 rotations (ExpParam.proto:195): two 32-slice
+    # tiles per x-filter GEMM and a 3-step rotation GEMM into two output
+    # tiles; every level on the matrix cores, two widths, three scales
+    ep = ExpParam(-180.0, 180.0, 48, 0.7, 1.3, 3, 0.0)
+    joints = [synth_joints(160), synth_joints(161)]
+    model = person_model(ep, joints[0])
+    sizes = [(112, 128), (96, 200)]
+    us = [synth_unary(162 + i, 10, 3, 48, h, w) for i, (h, w) in enumerate(sizes)]
+    say("48 rotations", method, sizes)
+    outs, (levels, used) = run_method(model, us, joints, method)
+    assert used == levels == 4
+    for i in range(len(sizes)):
+        worst, map_bad = errors_vs_f64(model, us[i], joints[i], outs[i])
+        say("  instance %d %s: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (i, sizes[i], worst, map_bad))
+        assert worst <= METHOD_L1[method]
+        assert not map_bad, map_bad

@@ -141,8 +141,67 @@ __device__ __forceinline__ void zero_halo(float *P, int R, int W, int kx, const 
 // window sliding along the row in two 16-byte LDS reads per chunk; writes
 // row y of T (row pitch tp, a multiple of 32: the outputs past W land in the
 // padding)
+// NCH chunks known at compile time: the chunk loop unrolls, so the sliding
+// window is renamed instead of copied
+template <int NCH>
+__device__ __forceinline__ void xfilter_rows_n(const float *P, const float *taps_x, float *T, int R, int tp,
+                                               long long plane, int y, const RowGeom &g) {
+  const int n_items = R * g.Wo;
+  const float inv_wo = 1.0f / (float)g.Wo;
+  for (int item = threadIdx.x; item < n_items; item += blockDim.x) {
+    int ro = (int)((float)item * inv_wo);
+    if (ro * g.Wo > item) --ro;
+    if ((ro + 1) * g.Wo <= item) ++ro;
+    const int q = item - ro * g.Wo;
+    // rows of P start on 32-byte boundaries (Pw is a multiple of 8)
+    const float4 *P4 = reinterpret_cast<const float4 *>(
+        __builtin_assume_aligned(P + ro * g.Pw + 8 * q, 16));
+    float a[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = 0.0f;
+    float4 w0 = P4[0], w1 = P4[1];
+    // the taps are re-read per item through the scalar cache rather than held
+    // in vector registers for the whole loop
+    const float *tx = taps_x;
+    asm volatile("" : "+s"(tx));
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const float4 w2 = P4[2 * c + 2], w3 = P4[2 * c + 3];
+      const float win[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                             w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+      float f[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f[u] = tx[8 * c + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = fmaf(win[i + u], f[u], a[i]);
+      w0 = w2;
+      w1 = w3;
+    }
+    float *trow = T + ro * plane + (long long)y * tp;
+    const int x0 = 8 * q;
+    *reinterpret_cast<float4 *>(trow + x0) = make_float4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<float4 *>(trow + x0 + 4) = make_float4(a[4], a[5], a[6], a[7]);
+  }
+}
+
 __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x, float *T, int R, int tp,
                                              long long plane, int y, const RowGeom &g) {
+  switch (g.nch) {  // half widths up to 43 (the filters of sigma <= 14.3 px)
+    case 1: xfilter_rows_n<1>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 2: xfilter_rows_n<2>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 3: xfilter_rows_n<3>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 4: xfilter_rows_n<4>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 5: xfilter_rows_n<5>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 6: xfilter_rows_n<6>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 7: xfilter_rows_n<7>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 8: xfilter_rows_n<8>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 9: xfilter_rows_n<9>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 10: xfilter_rows_n<10>(P, taps_x, T, R, tp, plane, y, g); return;
+    case 11: xfilter_rows_n<11>(P, taps_x, T, R, tp, plane, y, g); return;
+    default: break;
+  }
   const int n_items = R * g.Wo;
   const float inv_wo = 1.0f / (float)g.Wo;
   for (int item = threadIdx.x; item < n_items; item += blockDim.x) {
@@ -335,20 +394,20 @@ constexpr int kColRows = 16;
 constexpr int kColWaves = 4;
 constexpr int kColTileY = kColRows * kColWaves;
 
-__host__ __device__ __forceinline__ int col_padded_taps(int ky) { return (2 * ky + 1 + 7) & ~7; }
+__host__ __device__ __forceinline__ int col_padded_taps(int ky) { return (2 * ky + 1 + 3) & ~3; }
 __host__ __device__ __forceinline__ int col_tile_rows(int ky) { return kColTileY + col_padded_taps(ky); }
 // rows moved per thread and pass when a tile hands its halo to the next one
 constexpr int kColMove = 24;
 
-// 8 taps of the y filter starting at tap t; PH is the ring phase (t % 16)
-template <int PH>
-__device__ __forceinline__ void col_chunk8(float (&acc)[kColRows], float (&buf)[kColRows], const float *fy,
-                                           const float *tw, int t) {
-  float f[8];
+// N (8 or 4) taps of the y filter starting at tap t; PH is the ring phase (t % 16)
+template <int PH, int N>
+__device__ __forceinline__ void col_chunk(float (&acc)[kColRows], float (&buf)[kColRows], const float *fy,
+                                          const float *tw, int t) {
+  float f[N];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) f[u] = fy[t + u];
+  for (int u = 0; u < N; ++u) f[u] = fy[t + u];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < N; ++u) {
 #pragma unroll
     for (int i = 0; i < kColRows; ++i) acc[i] = fmaf(buf[(i + u + PH) & (kColRows - 1)], f[u], acc[i]);
     buf[(u + PH) & (kColRows - 1)] = tw[(t + u + kColRows) * kColTileX];
@@ -447,10 +506,19 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
       a[i] = 0.0f;
       buf[i] = tw[i * kColTileX];
     }
-    for (int t0 = 0; t0 < Lp; t0 += 2 * 8) {
-      col_chunk8<0>(a, buf, fy, tw, t0);
-      if (t0 + 8 >= Lp) break;
-      col_chunk8<8>(a, buf, fy, tw, t0 + 8);
+    // taps in 16s (the ring's period), then a tail of 4, 8 or 12 (the tap
+    // count is padded to a multiple of 4 with zero taps)
+    int t0 = 0;
+    for (; t0 + 16 <= Lp; t0 += 16) {
+      col_chunk<0, 8>(a, buf, fy, tw, t0);
+      col_chunk<8, 8>(a, buf, fy, tw, t0 + 8);
+    }
+    const int rem = Lp - t0;
+    if (rem >= 8) {
+      col_chunk<0, 8>(a, buf, fy, tw, t0);
+      if (rem == 12) col_chunk<8, 4>(a, buf, fy, tw, t0 + 8);
+    } else if (rem == 4) {
+      col_chunk<0, 4>(a, buf, fy, tw, t0);
     }
 
 #pragma unroll

@@ -473,8 +473,10 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
     const bool more = ty0 + kColTileY < H;
     const int y0 = ty0 + wave * kColRows;
     const unsigned eb0 = (unsigned)(y0 * W + xo) * 4u;
-    // the epilogue's operands (unary, stored belief), loaded now so that their
-    // latency hides behind the filter
+    // the unary of the part written, loaded now so that its latency hides
+    // behind the filter; the stored belief of the five-grid downward epilogue
+    // is loaded after the filter (held through it, its 16 registers make the
+    // kernel spill at 128 VGPRs)
     float pu[kColRows], pb[kColRows];
     if (kNeedU || kNeedB) {
       if (ty0 + kColTileY <= H) {
@@ -482,14 +484,12 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
         for (int i = 0; i < kColRows; ++i) {
           const unsigned eb = eb0 + (unsigned)i * Wb;
           if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
-          if (kNeedB) pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
         }
       } else {
 #pragma unroll
         for (int i = 0; i < kColRows; ++i) {
           const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
           if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
-          if (kNeedB) pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
         }
       }
     }
@@ -521,6 +521,13 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
       col_chunk<0, 4>(a, buf, fy, tw, t0);
     }
 
+    if (kNeedB) {
+#pragma unroll
+      for (int i = 0; i < kColRows; ++i) {
+        const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
+        pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
+      }
+    }
 #pragma unroll
     for (int i = 0; i < kColRows; ++i) {
       const int y = y0 + i;
@@ -559,10 +566,11 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
 }
 
 // HI: the highest destination kind among the launch's jobs (the kernel holds
-// only the epilogues up to it: 100-123 VGPRs below the five-grid downward
+// only the epilogues up to it: 102-127 VGPRs below the five-grid downward
 // one).  At most 128 VGPRs (4 waves per SIMD instead of 3): the five-grid
-// epilogue spills a few registers, which costs less than the lost occupancy
-// (9 % faster on 300x300x24 grids).
+// epilogue fits without spilling vector registers once its stored belief is
+// loaded after the filter (2.16 -> 1.33 ms per launch on 32 300x300x24
+// grids; unbounded, at 156 VGPRs and 3 waves per SIMD, it takes 1.65 ms).
 template <int HI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_colpass(
     Bases b, const MsgJob *__restrict__ jobs, const float *__restrict__ taps, const int *__restrict__ shifts, int R,

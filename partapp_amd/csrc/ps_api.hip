@@ -41,13 +41,14 @@ hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, hipStream_t s);
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms);
 size_t colpass_mfma_lds_bytes(int KK, int terms);
-size_t colpass_f16_lds_bytes(int KK);
+size_t colpass_f16_lds_bytes(int KK, int H);
 hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes);
 hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                                const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
-                               size_t lds_bytes, hipStream_t s);
+                               size_t lds_bytes, bool leaf, hipStream_t s);
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
-                               int R, int terms, bool t16, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s);
+                               int R, int terms, bool t16, int epi_hi, int max_w, size_t lds_bytes, bool leaf,
+                               hipStream_t s);
 hipError_t launch_map_decode(const Bases &b, const MapJob *jobs, int n, ps_map *out, hipStream_t s);
 }  // namespace ps
 
@@ -71,6 +72,7 @@ struct WidthClass {
   int epi_hi = kEpiUpRoot;  // highest epilogue kind of its jobs
   int mfma_rows = 0;        // tuned rows per block of the BF16 x pass (0: the plan's)
   int row_threads = 0;      // tuned threads per block of the FP32 row pass (0: one per column)
+  bool leaf = false;        // a leaf's message among its jobs (its source is a unary)
   // the jobs of each instance: consecutive within the class (the level's
   // jobs keep instance order within a class); (instance, first job relative
   // to the level, count), by instance
@@ -80,6 +82,7 @@ struct WidthClass {
 // record the instance runs of a class of a level's jobs v
 void class_runs(WidthClass &c, const std::vector<MsgJob> &v) {
   for (int j = c.start; j < c.start + c.count; ++j) {
+    if (v[j].src_kind == kSrcUnary) c.leaf = true;
     if (c.run_inst.empty() || c.run_inst.back() != v[j].inst) {
       c.run_inst.push_back(v[j].inst);
       c.run_first.push_back(j);
@@ -181,6 +184,10 @@ struct ps_plan {
   unsigned *not_fallback = nullptr;     // per instance: 1 - fallback flag (the pruned pass's active set)
   unsigned *flags_host = nullptr;       // pinned copy of the guard flags (eager execution)
   int max_children = 0;                 // root children (MaxJobs per item)
+  // every level runs on the BF16 passes (no off-diagonal job): with an fp16
+  // row-pass output no kernel reads the leaves' maxima (the row pass takes a
+  // per-row exponent instead), so those sweeps skip k_max
+  bool all_mfma = false;
   // the flagged instances' jobs gathered for the guard's FP32 re-sweep:
   // host copies of the tables, a pinned staging block and its device twin
   std::vector<MsgJob> h_jobs;
@@ -748,7 +755,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
               L.mfma_row_lds[t] = std::max(L.mfma_row_lds[t], rowpass_mfma_lds_bytes(R, j.W, j.kx, j.kk_x, t + 1));
               L.mfma_col_lds[t] = std::max(L.mfma_col_lds[t], colpass_mfma_lds_bytes(j.kk_y, t + 1));
             }
-            L.mfma_col16_lds = std::max(L.mfma_col16_lds, colpass_f16_lds_bytes(j.kk_y));
+            L.mfma_col16_lds = std::max(L.mfma_col16_lds, colpass_f16_lds_bytes(j.kk_y, j.H));
           }
         }
       }
@@ -771,6 +778,10 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   pl->n_roots = (int)B.roots.size();
   pl->root_children = B.roots.empty() ? 0 : B.roots[0].K;
   pl->n_maxes = (int)B.maxes.size();
+  pl->all_mfma = true;
+  for (const auto *lv : {&pl->up, &pl->down})
+    for (const Level &L : *lv)
+      if (!L.mfma_ok || L.n_diag != L.count) pl->all_mfma = false;
   pl->max_children = K;
   pl->n_keys = items * 2 * P;
   pl->n_map = items * P;
@@ -1055,7 +1066,7 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
   long ev = profile_begin(pp, 0, s);
   hipError_t e = launch_reset(b, pl->n_keys, pl->n_map, pl->n_inst, clear_flags, pl->n_inst, s);
   profile_end(pp, ev, s);
-  if (e == hipSuccess) {
+  if (e == hipSuccess && !((terms == 1 || terms == 2) && pl->all_mfma)) {
     ev = profile_begin(pp, 1, s);
     e = launch_max(b, pl->maxes, pl->n_maxes, pl->R, pl->max_hw, s);
     profile_end(pp, ev, s);
@@ -1074,7 +1085,7 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
       if (mf)
         e = launch_rowpass_mfma(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->frags, pl->R, terms,
                                 c.mfma_rows > 0 ? c.mfma_rows : pl->mfma_rows, c.max_h, c.max_w,
-                                L.mfma_row_lds[ti], s);
+                                L.mfma_row_lds[ti], c.leaf, s);
       else
         e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s,
                            c.row_threads);
@@ -1095,7 +1106,7 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
       const long ev2 = profile_begin(pp, 3, s);
       if (mf)
         e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, col_terms, terms != 3,
-                                c.epi_hi, c.max_w, terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], s);
+                                c.epi_hi, c.max_w, terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], c.leaf, s);
       else
         e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s);
       profile_end(pp, ev2, s);

@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "ps_device.hpp"
 #include "ps_internal.hpp"
@@ -137,7 +138,7 @@ struct RowTiles {
   static constexpr int NRES = R == 24 ? 12 : 24;  // outputs per lane and column tile
 };
 
-template <int R, int TERMS, bool T16>
+template <int R, int TERMS, bool T16, bool ROWMAX>
 __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restrict__ jobs,
                                                   const int *__restrict__ shifts, const uint4 *__restrict__ frags,
                                                   int rows) {
@@ -160,6 +161,7 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   uint4 *frr = fr + nf * KK * 64;  // [nr][MT][KS][64] rotation fragments
   unsigned short *Ph = reinterpret_cast<unsigned short *>(frr + nr * MT * KS * 64);
   unsigned short *Pl = Ph + R * pitch;
+  unsigned *wkey = reinterpret_cast<unsigned *>(Ph + (TERMS >= 2 ? 2 : 1) * R * pitch);  // [waves]
 
   const uint4 *gf = frags + J.frag_x;
   for (int i = threadIdx.x; i < nf * KK * 64; i += blockDim.x) fr[i] = gf[i];
@@ -177,8 +179,7 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
 
   // exp(b - M) = 2^(b log2 e - M log2 e)
   constexpr float kLog2e = 1.4426950408889634f;
-  const float M = key2f(b.max_keys[J.src_key]);
-  const float mM = -M * kLog2e;
+  const float M = J.src_kind == kSrcGrid || !T16 ? key2f(b.max_keys[J.src_key]) : 0.0f;
   const float *src = cptr(b, J.src);
   const bool zero_src = J.src_kind == kSrcZero;
   const UnaryMask mk = J.src_mask;
@@ -222,7 +223,9 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
       v[ro] = src[rs * HW + yy * W + xx];
     }
   };
-  auto consume = [&](int y, const float (&v)[R]) {
+  // exp(b - M) = 2^(b log2 e - sub) of row y into the LDS rows, sub = M log2 e
+  // (M the source grid's maximum) or, for an fp16 output, the row's exponent
+  auto consume = [&](int y, const float (&v)[R], float sub) {
     int ox[R], oy[R];
     offsets(ox, oy);
     const int x = lane_column();
@@ -230,8 +233,8 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
     if (zero_src) {
       // a leaf without a detector keeps the zero log posterior it was
       // allocated with (no unary added, objectdetect_findrot.cpp:652-654):
-      // exp(0 - M) = 1 (its max M is 0) wherever the warp has a source
-      // pixel, exp(LOG_ZERO) = 0 elsewhere
+      // exp(0 - M) = 1 (its max M is 0, and its row exponent) wherever the
+      // warp has a source pixel, exp(LOG_ZERO) = 0 elsewhere
 #pragma unroll
       for (int ro = 0; ro < R; ++ro) {
         const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)(x + ox[ro]) < (unsigned)W;
@@ -246,14 +249,14 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
         const bool rot_masked = (unsigned)rs < (unsigned)R && ((mk.rot_bits >> (rs & 63)) & 1ull);
         const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)xx < (unsigned)W && !rot_masked &&
                         xx >= mk.strip && xx < W - mk.strip;
-        e[ro] = __builtin_amdgcn_exp2f(fmaf(ok ? v[ro] : kLogZero, kLog2e, mM));
+        e[ro] = __builtin_amdgcn_exp2f(fmaf(ok ? v[ro] : kLogZero, kLog2e, -sub));
       }
     } else {
 #pragma unroll
       for (int ro = 0; ro < R; ++ro) {
         const int xx = x + ox[ro];
         const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)xx < (unsigned)W;
-        e[ro] = __builtin_amdgcn_exp2f(fmaf(ok ? v[ro] : kLogZero, kLog2e, mM));
+        e[ro] = __builtin_amdgcn_exp2f(fmaf(ok ? v[ro] : kLogZero, kLog2e, -sub));
       }
     }
     if (x < W) {
@@ -284,6 +287,9 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   const unsigned ob0 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t0 + (lane & 31)) * eb;
   const unsigned ob1 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t1 + (lane & 31)) * eb;
   char *Tbase = reinterpret_cast<char *>(mptr(b, J.tmp));
+  // fp16 rows take the first half of the output grid's bytes; their
+  // exponents follow them
+  int *rexp = reinterpret_cast<int *>(Tbase + (size_t)R * H * tp * 2);
 
   // x filter and rotation of one 32-column tile; res[i] is output slice
   // 8 (i / 4) + 4 h + i % 4 (< 24) of column 32 t + lane % 32 (R = 48: res[i]
@@ -369,17 +375,58 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
     }
   };
 
+  // T16: row y is stored as 2^(b log2 e - e_y) and e_y beside the output.
+  // A source the sweep wrote itself (every source but a leaf's) has its
+  // maximum M in the max keys: e_y = ceil(M log2 e) for every row.  A leaf's
+  // unary (whose maximum no kernel computes) takes e_y = ceil(m log2 e), m
+  // the largest value row y's loads read (each load reads a value of the
+  // grid, masked or not: a bound of the row's unmasked values), from the
+  // waves' maxima (order-preserving keys, one LDS slot per wave) published
+  // before the barrier that ends the previous row; a leaf without a
+  // detector has e_y = 0 (its values are 0 or LOG_ZERO).
+  const bool row_max = ROWMAX && J.src_kind == kSrcUnary;
+  const int e_grid = T16 && J.src_kind == kSrcGrid ? (int)ceilf(M * kLog2e) : 0;
+  const int nw = (int)(blockDim.x >> 6);
+  auto publish = [&](const float (&v)[R]) {
+    float mx = v[0];
+#pragma unroll
+    for (int ro = 1; ro < R; ++ro) mx = fmaxf(mx, v[ro]);
+    // wave maximum; the lane index is re-read through an empty asm so that
+    // the six exchange addresses are not hoisted out of the row loop
+    int self = lane_column() & 63;
+    unsigned k = f2key(mx);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = max(k, (unsigned)__builtin_amdgcn_ds_bpermute((self ^ o) << 2, (int)k));
+    if (lane == 0) wkey[wave] = k;
+  };
+  auto row_exponent = [&]() -> int {
+    if (!row_max) return e_grid;
+    unsigned k = 0u;
+    for (int w = 0; w < nw; ++w) k = max(k, wkey[w]);
+    return (int)ceilf(key2f(k) * kLog2e);
+  };
+
   // Row y: its values into LDS, the loads of row y + 1 issued (in flight
   // through the GEMMs), then both tiles and their stores.
   float v[R];
   gather(y0, v);
+  if (row_max) {
+    publish(v);
+    __syncthreads();
+  }
   for (int y = y0; y < y1; ++y) {
     // per-slice scalars are recomputed every row rather than hoisted
     asm volatile("" : "+s"(so), "+s"(HW), "+s"(delta), "+s"(slice_b));
-    consume(y, v);
+    if (T16) {
+      const int ey = row_exponent();
+      if (threadIdx.x == 0) rexp[y] = ey;
+      consume(y, v, (float)ey);
+    } else {
+      consume(y, v, M * kLog2e);
+    }
     __builtin_amdgcn_sched_barrier(0);
     gather(min(y + 1, y1 - 1), v);
-    __syncthreads();  // row y is in LDS
+    __syncthreads();  // row y is in LDS; every wave has read its maxima
     char *Trow = Tbase + (long long)y * tp * eb;
     float res[NRES];
     if (2 * wave < ntile) {  // waves past the grid's width (a narrower job of the launch) idle
@@ -391,21 +438,23 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
       tile(t1, res);
       store_tile(ob1, Trow, res);
     }
+    __builtin_amdgcn_sched_barrier(0);
+    if (row_max && y + 1 < y1) publish(v);
     __syncthreads();  // P is rewritten by the next row
   }
 }
 
-template <int R, int TERMS, bool T16>
+// ROWMAX: the launch holds leaf messages, whose rows take their own exponents
+template <int R, int TERMS, bool T16, bool ROWMAX>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(7))) void k_rowpass_mfma(
     Bases b, const MsgJob *__restrict__ jobs, const int *__restrict__ shifts, const uint4 *__restrict__ frags, int rows) {
-  rowpass_mfma_body<R, TERMS, T16>(b, jobs, shifts, frags, rows);
+  rowpass_mfma_body<R, TERMS, T16, ROWMAX>(b, jobs, shifts, frags, rows);
 }
-// 48 slices hold twice the accumulators: no occupancy target
-template <int TERMS, bool T16>
-__global__ __launch_bounds__(512) void k_rowpass_mfma48(Bases b, const MsgJob *__restrict__ jobs,
-                                                        const int *__restrict__ shifts,
-                                                        const uint4 *__restrict__ frags, int rows) {
-  rowpass_mfma_body<48, TERMS, T16>(b, jobs, shifts, frags, rows);
+// 48 slices hold twice the accumulators: at most 128 VGPRs, 4 waves per SIMD
+template <int TERMS, bool T16, bool ROWMAX>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_rowpass_mfma48(
+    Bases b, const MsgJob *__restrict__ jobs, const int *__restrict__ shifts, const uint4 *__restrict__ frags, int rows) {
+  rowpass_mfma_body<48, TERMS, T16, ROWMAX>(b, jobs, shifts, frags, rows);
 }
 
 // ---------------------------------------------------- column pass (y filter)
@@ -642,7 +691,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
 // skips its GEMM and epilogue operands and is written as LOG_ZERO (zero
 // probability: the pruned model sets those states' unaries to -infinity).
 // The ring keeps staging every tile, so the next kept tile finds its rows.
-template <int EPI, bool PRUNE>
+template <int EPI, bool PRUNE, bool RS>
 __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &J, const uint4 *__restrict__ frags,
                                                   const int *__restrict__ shifts, int R, int r, int xt, uint4 *lds) {
   constexpr bool kNeedU = Epi<EPI>::kNeedU;
@@ -663,7 +712,35 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   for (int i = threadIdx.x; i < 16 * q8; i += blockDim.x) fr[i] = gf[i];
   const int ox = shifts[J.shifts + 2 * R + r];
   const int oy = shifts[J.shifts + 3 * R + r];
-  const float M = key2f(b.max_keys[J.src_key]);
+  // the row pass stored row y as 2^(v log2 e - e_y) (times 2^t16_shift), e_y
+  // beside the output.  RS (a leaf message in the launch: rows with their own
+  // exponents): the rows are brought to the message's largest exponent emax
+  // as they are staged, by the exact power-of-two scales 2^(e_y - emax) (0
+  // below 2^-24), kept in LDS as fp16 pairs: sp[1 + i] scales rows i and
+  // i + 1 (clamped into the grid; i = -1 .. H - 1), the pairs the ring
+  // stores.  Without RS every row of a message has the same exponent.
+  const int *rexp = reinterpret_cast<const int *>(reinterpret_cast<const char *>(cptr(b, J.tmp)) +
+                                                  (size_t)R * H * tp * 2);
+  int emax;
+  unsigned *sp = nullptr;
+  if (RS && J.src_kind == kSrcUnary) {
+    int *xk = reinterpret_cast<int *>(Rg + 64 * cs);  // [4] the waves' largest row exponents
+    sp = reinterpret_cast<unsigned *>(xk + 4);         // [H + 1] fp16 scale pairs
+    int em = INT_MIN;
+    for (int i = threadIdx.x; i < H; i += blockDim.x) em = max(em, rexp[i]);
+    const unsigned k = wave_max_u32((unsigned)em ^ 0x80000000u);
+    if (lane == 0) xk[wave] = (int)(k ^ 0x80000000u);
+    __syncthreads();
+    emax = max(max(xk[0], xk[1]), max(xk[2], xk[3]));
+    auto scale_bits = [&](int row) -> unsigned {
+      const int d = rexp[min(max(row, 0), H - 1)] - emax;
+      return d >= -14 ? (unsigned)(d + 15) << 10 : (d >= -24 ? 1u << (d + 24) : 0u);
+    };
+    for (int i = (int)threadIdx.x - 1; i < H; i += blockDim.x) sp[1 + i] = scale_bits(i) | scale_bits(i + 1) << 16;
+    __syncthreads();
+  } else {
+    emax = rexp[0];
+  }
   const int HW = H * W;
   float *dst = mptr(b, J.dst) + r * HW;
   const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
@@ -692,11 +769,16 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   auto put16 = [&](int row0, int prow, const h2_t (&v)[8]) {
     const bool all = lcol_ok && row0 >= 0 && row0 + 16 <= H;
     h2_t *dstp = reinterpret_cast<h2_t *>(Rg + lane * cs + prow);
+    unsigned s8[8];  // the scales, read before any ring store (which could alias them)
+    if (RS && sp) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s8[k] = sp[1 + min(max(row0 + 2 * k, -1), H - 1)];
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const bool o0 = all || (lcol_ok && row0 + 2 * k >= 0 && row0 + 2 * k < H);
       const bool o1 = all || (lcol_ok && row0 + 2 * k + 1 >= 0 && row0 + 2 * k + 1 < H);
-      h2_t m = v[k];
+      h2_t m = RS && sp ? v[k] * __builtin_bit_cast(h2_t, s8[k]) : v[k];
       m.x = o0 ? m.x : (_Float16)0.0f;
       m.y = o1 ? m.y : (_Float16)0.0f;
       dstp[k] = m;
@@ -730,9 +812,10 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   // read from the copy shifted by c mod 8 at the aligned element 16 kk + 8 (c / 8)
   const int ca = 8 * h + 32 - n;
   const uint4 *fa = fr + (ca & 7) * q8 + (ca >> 3);
-  // log(v 2^-s) + M = log2(v) ln 2 + (M - s ln 2), s the row pass's fp16 shift
+  // log(v 2^-s) + emax ln 2 = log2(v) ln 2 + (emax - s) ln 2, s the row
+  // pass's fp16 shift
   constexpr float kLn2 = 0.693147180559945309f;
-  const float cM = fmaf(-(float)J.t16_shift, kLn2, M);
+  const float cM = (float)(emax - J.t16_shift) * kLn2;
   const int xo = xin ? x : W - 1;
   unsigned Wb = (unsigned)W * 4u;
 
@@ -816,7 +899,7 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
 // only the kinds up to HI, so a launch without the five-grid downward
 // epilogue is not held to its register count (UpRoot 86, UpNodeU 94,
 // DownLeaf 98, DownMid 116 VGPRs + AGPRs).
-template <int HI, bool PRUNE>
+template <int HI, bool PRUNE, bool RS>
 __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__restrict__ jobs,
                                                      const uint4 *__restrict__ frags, const int *__restrict__ shifts,
                                                      int R, int xtiles) {
@@ -830,15 +913,15 @@ __global__ __launch_bounds__(256) void k_colpass_f16(Bases b, const MsgJob *__re
   if (xt * 64 >= J.W) return;
   const int kind = epi_kind(J);
   if (HI == kEpiUpRoot || kind == kEpiUpRoot)
-    colpass_f16_strip<kEpiUpRoot, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiUpRoot, PRUNE, RS>(b, J, frags, shifts, R, r, xt, lds);
   else if (HI == kEpiUpNode || kind == kEpiUpNode)
-    colpass_f16_strip<kEpiUpNode, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiUpNode, PRUNE, RS>(b, J, frags, shifts, R, r, xt, lds);
   else if (HI == kEpiUpNodeU || kind == kEpiUpNodeU)
-    colpass_f16_strip<kEpiUpNodeU, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiUpNodeU, PRUNE, RS>(b, J, frags, shifts, R, r, xt, lds);
   else if (HI == kEpiDownLeaf || kind == kEpiDownLeaf)
-    colpass_f16_strip<kEpiDownLeaf, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiDownLeaf, PRUNE, RS>(b, J, frags, shifts, R, r, xt, lds);
   else
-    colpass_f16_strip<kEpiDownMid, PRUNE>(b, J, frags, shifts, R, r, xt, lds);
+    colpass_f16_strip<kEpiDownMid, PRUNE, RS>(b, J, frags, shifts, R, r, xt, lds);
 }
 
 template <int TERMS>
@@ -870,95 +953,133 @@ __global__ __launch_bounds__(256) void k_colpass_mfma(Bases b, const MsgJob *__r
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
   const int rot = 2 * ((R + 31) / 32) * ((R + 15) / 16);  // hi / lo rotation fragments per lane
-  return 16 * ((size_t)nf * KK + rot) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK);
+  return 16 * ((size_t)nf * KK + rot) * 64 + 2 * (size_t)nd * R * mfma_row_pitch(W, kx, KK) + 64;
 }
 size_t colpass_mfma_lds_bytes(int KK, int terms) {
   const int nf = terms == 3 ? 2 : 1, nd = terms >= 2 ? 2 : 1;
   return 16 * (size_t)nf * KK * 64 + 2 * (size_t)nd * 64 * mfma_ring_pitch(KK) + 8 * 64;
 }
 
-size_t colpass_f16_lds_bytes(int KK) {
-  return 2 * 8 * 2 * (size_t)f16_tap_copy_len(KK) + 2 * (size_t)64 * mfma_ring_pitch(KK);
+size_t colpass_f16_lds_bytes(int KK, int H) {
+  return 2 * 8 * 2 * (size_t)f16_tap_copy_len(KK) + 2 * (size_t)64 * mfma_ring_pitch(KK) + 16 + 4 * ((size_t)H + 1);
 }
 
 hipError_t raise_lds_limit(const void *f, size_t bytes);
 
-hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
-  auto set = [](const void *f, size_t bytes) { return raise_lds_limit(f, bytes); };
-  hipError_t e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1, true>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma48<1, true>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma48<2, true>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_rowpass_mfma48<3, false>), row_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, false>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode, false>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU, false>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf, false>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf, true>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid, false>), col_bytes);
-  if (e == hipSuccess) e = set(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid, true>), col_bytes);
+template <bool PRUNE, bool RS>
+static hipError_t colpass_f16_set_lds_limit(size_t bytes) {
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpRoot, PRUNE, RS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNode, PRUNE, RS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass_f16<kEpiUpNodeU, PRUNE, RS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownLeaf, PRUNE, RS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass_f16<kEpiDownMid, PRUNE, RS>), bytes);
   return e;
 }
 
+template <bool ROWMAX>
+static hipError_t rowpass_mfma_set_lds_limit(size_t bytes) {
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 1, true, ROWMAX>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 2, true, ROWMAX>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_mfma48<1, true, ROWMAX>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_mfma48<2, true, ROWMAX>), bytes);
+  return e;
+}
+
+hipError_t mfma_set_lds_limit(size_t row_bytes, size_t col_bytes) {
+  hipError_t e = rowpass_mfma_set_lds_limit<false>(row_bytes);
+  if (e == hipSuccess) e = rowpass_mfma_set_lds_limit<true>(row_bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_mfma<24, 3, false, false>), row_bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_mfma48<3, false, false>), row_bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass_mfma<3>), col_bytes);
+  if (e == hipSuccess) e = colpass_f16_set_lds_limit<false, false>(col_bytes);
+  if (e == hipSuccess) e = colpass_f16_set_lds_limit<false, true>(col_bytes);
+  if (e == hipSuccess) e = colpass_f16_set_lds_limit<true, false>(col_bytes);
+  if (e == hipSuccess) e = colpass_f16_set_lds_limit<true, true>(col_bytes);
+  return e;
+}
+
+template <int R, bool ROWMAX>
+static void launch_rowpass_t16(int terms, dim3 grid, int nt, size_t lds_bytes, hipStream_t s, const Bases &b,
+                               const MsgJob *jobs, const int *shifts, const uint4 *frags, int rows) {
+  if (R == 48) {
+    if (terms == 2)
+      hipLaunchKernelGGL((k_rowpass_mfma48<2, true, ROWMAX>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    else
+      hipLaunchKernelGGL((k_rowpass_mfma48<1, true, ROWMAX>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+  } else {
+    if (terms == 2)
+      hipLaunchKernelGGL((k_rowpass_mfma<24, 2, true, ROWMAX>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+    else
+      hipLaunchKernelGGL((k_rowpass_mfma<24, 1, true, ROWMAX>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+  }
+}
+
+// leaf: the jobs include a leaf's message (a source without a grid maximum)
 hipError_t launch_rowpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                                const uint4 *frags, int R, int terms, int rows, int max_h, int max_w,
-                               size_t lds_bytes, hipStream_t s) {
+                               size_t lds_bytes, bool leaf, hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
   // one lane per column: the plan only routes grids of width <= 512 here
   if ((R != 24 && R != 48) || terms < 1 || terms > 3 || max_w > 512 || rows < 1) return hipErrorInvalidValue;
   dim3 grid((max_h + rows - 1) / rows, n_jobs);
   const int nt = std::min(512, std::max(64, (max_w + 63) / 64 * 64));
-  if (R == 48) {
-    if (terms == 3)
-      hipLaunchKernelGGL((k_rowpass_mfma48<3, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
-    else if (terms == 2)
-      hipLaunchKernelGGL((k_rowpass_mfma48<2, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+  if (terms == 3) {
+    if (R == 48)
+      hipLaunchKernelGGL((k_rowpass_mfma48<3, false, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
     else
-      hipLaunchKernelGGL((k_rowpass_mfma48<1, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
-    return hipGetLastError();
+      hipLaunchKernelGGL((k_rowpass_mfma<24, 3, false, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
+  } else if (R == 48) {
+    if (leaf)
+      launch_rowpass_t16<48, true>(terms, grid, nt, lds_bytes, s, b, jobs, shifts, frags, rows);
+    else
+      launch_rowpass_t16<48, false>(terms, grid, nt, lds_bytes, s, b, jobs, shifts, frags, rows);
+  } else {
+    if (leaf)
+      launch_rowpass_t16<24, true>(terms, grid, nt, lds_bytes, s, b, jobs, shifts, frags, rows);
+    else
+      launch_rowpass_t16<24, false>(terms, grid, nt, lds_bytes, s, b, jobs, shifts, frags, rows);
   }
-  if (terms == 3)
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 3, false>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
-  else if (terms == 2)
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 2, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
-  else
-    hipLaunchKernelGGL((k_rowpass_mfma<24, 1, true>), grid, dim3(nt), lds_bytes, s, b, jobs, shifts, frags, rows);
   return hipGetLastError();
 }
 
+template <bool PRUNE, bool RS>
+static void launch_colpass_f16(int epi_hi, dim3 grid, size_t lds_bytes, hipStream_t s, const Bases &b,
+                               const MsgJob *jobs, const uint4 *frags, const int *shifts, int R, int xtiles) {
+  const dim3 blk(256);
+  switch (epi_hi) {
+    case kEpiUpRoot: hipLaunchKernelGGL((k_colpass_f16<kEpiUpRoot, PRUNE, RS>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+    case kEpiUpNode: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNode, PRUNE, RS>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+    case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNodeU, PRUNE, RS>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+    case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass_f16<kEpiDownLeaf, PRUNE, RS>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+    default: hipLaunchKernelGGL((k_colpass_f16<kEpiDownMid, PRUNE, RS>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+  }
+}
+
 // t16: the row pass stored its output as fp16 (row-pass terms 1 and 2);
-// epi_hi: the highest epilogue kind (EpiKind) among the jobs
+// epi_hi: the highest epilogue kind (EpiKind) among the jobs; leaf: the jobs
+// include a leaf's message (fp16 rows with their own exponents)
 hipError_t launch_colpass_mfma(const Bases &b, const MsgJob *jobs, int n_jobs, const uint4 *frags, const int *shifts,
-                               int R, int terms, bool t16, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s) {
+                               int R, int terms, bool t16, int epi_hi, int max_w, size_t lds_bytes, bool leaf,
+                               hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
   if (terms < 1 || terms > 3 || (!t16 && terms != 3) || epi_hi < kEpiUpRoot || epi_hi > kEpiDownMid)
     return hipErrorInvalidValue;
   const int xtiles = (max_w + 63) / 64;
   dim3 grid(R * xtiles, n_jobs);
-  const dim3 blk(256);
-  if (t16 && b.keep) {  // a pruned execution
-    switch (epi_hi) {
-      case kEpiUpRoot: hipLaunchKernelGGL((k_colpass_f16<kEpiUpRoot, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiUpNode: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNode, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNodeU, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass_f16<kEpiDownLeaf, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      default: hipLaunchKernelGGL((k_colpass_f16<kEpiDownMid, true>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
+  if (t16) {  // fp16 operands: exact data, split taps, whatever the method's terms
+    if (b.keep) {  // a pruned execution
+      if (leaf)
+        launch_colpass_f16<true, true>(epi_hi, grid, lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+      else
+        launch_colpass_f16<true, false>(epi_hi, grid, lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+    } else if (leaf) {
+      launch_colpass_f16<false, true>(epi_hi, grid, lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+    } else {
+      launch_colpass_f16<false, false>(epi_hi, grid, lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
     }
-  } else if (t16) {  // fp16 operands: exact data, split taps, whatever the method's terms
-    switch (epi_hi) {
-      case kEpiUpRoot: hipLaunchKernelGGL((k_colpass_f16<kEpiUpRoot, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiUpNode: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNode, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass_f16<kEpiUpNodeU, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass_f16<kEpiDownLeaf, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-      default: hipLaunchKernelGGL((k_colpass_f16<kEpiDownMid, false>), grid, blk, lds_bytes, s, b, jobs, frags, shifts, R, xtiles); break;
-    }
-  } else  // an FP32 row-pass output (3-term row pass): bf16 hi / lo split of data and taps
+  } else {  // an FP32 row-pass output (3-term row pass): bf16 hi / lo split of data and taps
     hipLaunchKernelGGL(k_colpass_mfma<3>, grid, dim3(256), lds_bytes, s, b, jobs, frags, shifts, R, xtiles);
+  }
   return hipGetLastError();
 }
 

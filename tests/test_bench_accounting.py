@@ -51,6 +51,8 @@ def test_committed_column_pass_traffic_is_consistent():
     assert got == int(per_launch)
     assert src.endswith("traffic_colpass_bf16mix_32x300x300x24.txt")
     # within 15 % of the algorithmic bytes: 48 grids of 32 x 300 x 300 x 24
-    # states (4 bytes) per sweep over the sweep's launches
-    alg = 48 * 32 * 300 * 300 * 24 * 4 / len(fetch)
+    # states (4 bytes) per sweep over the sweep's 4 launches (the file lists
+    # whole sweeps)
+    assert len(fetch) % 4 == 0
+    alg = 48 * 32 * 300 * 300 * 24 * 4 / 4
     assert 1.0 <= per_launch / alg < 1.15

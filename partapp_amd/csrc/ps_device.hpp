@@ -106,8 +106,13 @@ __device__ __forceinline__ bool masked(const UnaryMask &m, int r, int x, int W) 
   return ((m.rot_bits >> r) & 1ull) || x < m.strip || x >= W - m.strip;
 }
 
-// log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166)
-__device__ __forceinline__ float log0(float v) { return v == 0.0f ? kLogZero : logf(v); }
+// log of a filter response, 0 -> LOG_ZERO (multi_array_op.hpp:154-166), on
+// the hardware log2 (v_log_f32, about 1 ulp): a response the hardware takes
+// for 0 (below 2^-126 where it flushes) is LOG_ZERO too
+__device__ __forceinline__ float log0(float v) {
+  const float l = __builtin_amdgcn_logf(v);
+  return l > -INFINITY ? l * 0.693147180559945309f : kLogZero;
+}
 
 // The caller's accumulation of a column pass specialised per destination
 // kind (EpiKind), so that a kernel's loads for it are unconditional

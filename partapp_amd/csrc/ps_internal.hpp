@@ -8,6 +8,7 @@
 namespace ps {
 
 constexpr float kLogZero = -1e6f;  // LOG_ZERO, boost_math.h:23
+constexpr float kLog2eF = 1.4426950408889634f;  // log2(e)
 constexpr int kMaxRot = 64;        // rotation masks are 64-bit words
 constexpr int kMaxRootChildren = 16;
 constexpr int kMaxTaps = 1000;     // multi_array_filter.hpp:243-244

@@ -289,12 +289,14 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
 #pragma unroll
       for (int ro = 0; ro < R; ++ro) e[ro] = src[off[ro] < 0 ? 0 : off[ro]];
     }
-    // masks, then exp(b - M) (:362-365)
+    // masks, then exp(b - M) (:362-365) as 2^((b - M) log2 e) on the hardware
+    // exp2 (values below 2^-126 flush to zero, where the reference's expf
+    // returns subnormals)
 #pragma unroll
     for (int ro = 0; ro < R; ++ro) {
       float vv = e[ro];
       if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
-      e[ro] = off[ro] < 0 ? e_fill : expf(vv + negM);
+      e[ro] = off[ro] < 0 ? e_fill : __builtin_amdgcn_exp2f((vv + negM) * kLog2eF);
     }
     float o[R];
     if (J.rot_mode == kRotFilter) {
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
           if (J.src_kind == kSrcZero) v = 0.0f;
           else if (J.src_kind == kSrcUnary && masked(J.src_mask, rs, xx, W)) v = kLogZero;
           else v = src[rs * HW + (long long)yy * W + xx];
-          e = expf(v + negM);
+          e = __builtin_amdgcn_exp2f((v + negM) * kLog2eF);
         }
       }
       E[ro * Wa + x] = e;

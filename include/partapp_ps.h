@@ -161,13 +161,16 @@ int ps_plan_set_method(ps_plan *plan, int method);
  *                            default 4);
  *   PS_TUNE_FP32_ROW_THREADS threads per block of the FP32 row pass (a
  *                            multiple of 64 up to 512; 0 = one per column,
- *                            the default).
+ *                            the default);
+ *   PS_TUNE_FP32_COL_ROWS    output rows per lane of the FP32 column pass
+ *                            (8 or 16; 0 = 8, the default).
  * ps_plan_tune sets a knob for every launch class; ps_plan_tune_class for
  * one class.  A class is one (level, width class) row-pass launch of the
  * sweep, in execution order: ps_plan_classes returns their count and, per
- * class, (level, jobs, tallest grid, widest grid) in info[4 * c ..].  No
- * reference counterpart. */
-enum { PS_TUNE_MFMA_ROWS = 1, PS_TUNE_FP32_ROW_THREADS = 2 };
+ * class, (level, jobs, tallest grid, widest grid) in info[4 * c ..]; the
+ * column-pass knob of a class reaches the column-pass launches of its level
+ * and width class.  No reference counterpart. */
+enum { PS_TUNE_MFMA_ROWS = 1, PS_TUNE_FP32_ROW_THREADS = 2, PS_TUNE_FP32_COL_ROWS = 3 };
 int ps_plan_tune(ps_plan *plan, int knob, int value);
 int ps_plan_tune_class(ps_plan *plan, int knob, int cls, int value);
 int ps_plan_classes(const ps_plan *plan, int *n_classes, int *info, int max_classes);

@@ -262,6 +262,7 @@ class Plan:
 
     TUNE_MFMA_ROWS = 1
     TUNE_FP32_ROW_THREADS = 2
+    TUNE_FP32_COL_ROWS = 3
 
     def tune(self, knob, value, cls=None):
         """Set a launch-shape knob for every launch class, or for one."""
@@ -295,43 +296,49 @@ class Plan:
     def autotune(self, unary, marg, root, maps, method_is_fp32=False, reps=2, stream=None, max_classes=8):
         """Choose the launch shapes of the current method by timing whole
         sweeps on the caller's buffers (HIP events on the execution stream):
-        the BF16 methods' rows per x-pass block (candidates 2, 4, 8, 16), or
-        the FP32 row pass's threads per block (one per column, 128, 192, 256),
-        first one value for every launch class, then class by class
+        the BF16 methods' rows per x-pass block (candidates 2, 4, 8, 16); for
+        the FP32 method the row pass's threads per block (one per column,
+        128, 192, 256), then the column pass's output rows per lane (8, 16).
+        Per knob: first one value for every launch class, then class by class
         (coordinate descent: each class keeps the value that gave the fastest
         sweep with the others fixed; skipped above max_classes classes, where
-        it would take more sweeps than it can save).  Sets the choices and returns
-        {"knob": name, "all": {candidate: ms}, "per_class": [(class, value,
-        ms)], "ms": final ms per sweep}."""
+        it would take more sweeps than it can save).  Sets the choices and
+        returns {"knobs": [{"knob": name, "all": {candidate: ms},
+        "per_class": [(class, value, ms)]}], "ms": final ms per sweep}."""
         import torch
         s = stream if stream is not None else torch.cuda.current_stream()
         if method_is_fp32:
-            knob, name, cands = self.TUNE_FP32_ROW_THREADS, "fp32_row_threads", (0, 128, 192, 256)
+            knobs = [(self.TUNE_FP32_ROW_THREADS, "fp32_row_threads", (0, 128, 192, 256)),
+                     (self.TUNE_FP32_COL_ROWS, "fp32_col_rows_per_lane", (8, 16))]
         else:
-            knob, name, cands = self.TUNE_MFMA_ROWS, "mfma_rows_per_block", (2, 4, 8, 16)
-        times = {}
-        for c in cands:
-            self.tune(knob, c)
-            times[c] = self._sweep_ms(unary, marg, root, maps, s, reps)
-        best = min(times, key=times.get)
-        self.tune(knob, best)
-        cur = times[best]
-        per_class = []
+            knobs = [(self.TUNE_MFMA_ROWS, "mfma_rows_per_block", (2, 4, 8, 16))]
         classes = self.classes()
-        if 1 < len(classes) <= max_classes:
-            for k in range(len(classes)):
-                pick = best
-                for c in cands:
-                    if c == best:
-                        continue
-                    self.tune(knob, c, cls=k)
-                    t = self._sweep_ms(unary, marg, root, maps, s, reps)
-                    if t < cur:
-                        cur, pick = t, c
-                self.tune(knob, pick, cls=k)
-                per_class.append((k, pick, round(cur, 3)))
-        return {"knob": name, "all": {str(k): round(v, 3) for k, v in times.items()}, "per_class": per_class,
-                "ms": round(cur, 3)}
+        out = []
+        cur = None
+        for knob, name, cands in knobs:
+            times = {}
+            for c in cands:
+                self.tune(knob, c)
+                times[c] = self._sweep_ms(unary, marg, root, maps, s, reps)
+            best = min(times, key=times.get)
+            self.tune(knob, best)
+            cur = times[best]
+            per_class = []
+            if 1 < len(classes) <= max_classes:
+                for k in range(len(classes)):
+                    pick = best
+                    for c in cands:
+                        if c == best:
+                            continue
+                        self.tune(knob, c, cls=k)
+                        t = self._sweep_ms(unary, marg, root, maps, s, reps)
+                        if t < cur:
+                            cur, pick = t, c
+                    self.tune(knob, pick, cls=k)
+                    per_class.append((k, pick, round(cur, 3)))
+            out.append({"knob": name, "all": {str(k): round(v, 3) for k, v in times.items()},
+                        "per_class": per_class})
+        return {"knobs": out, "ms": round(cur, 3)}
 
     def set_profiling(self, enable=True, classes=None):
         """Bracket the launches with HIP events: every kernel class, or only

@@ -33,9 +33,9 @@ hipError_t rowpass_set_lds_limit(size_t bytes);
 hipError_t colpass_set_lds_limit(size_t bytes);
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
                           int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s, int threads = 0);
-size_t colpass_lds_bytes(int ky);
+size_t colpass_lds_bytes(int ky, int rows);
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s);
+                          int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows = 0);
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, hipStream_t s);
@@ -72,6 +72,7 @@ struct WidthClass {
   int epi_hi = kEpiUpRoot;  // highest epilogue kind of its jobs
   int mfma_rows = 0;        // tuned rows per block of the BF16 x pass (0: the plan's)
   int row_threads = 0;      // tuned threads per block of the FP32 row pass (0: one per column)
+  int col_rows = 0;         // tuned output rows per lane of the FP32 column pass (0: the default, 8)
   bool leaf = false;        // a leaf's message among its jobs (its source is a unary)
   // the jobs of each instance: consecutive within the class (the level's
   // jobs keep instance order within a class); (instance, first job relative
@@ -746,7 +747,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           L.od_hw = std::max(L.od_hw, (long long)j.H * j.W);
         } else {
           ++L.n_diag;
-          L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky));
+          L.col_lds = std::max(L.col_lds, colpass_lds_bytes(j.ky, 16));
           // (the BF16 passes address a row-pass output grid with 32-bit byte offsets)
           if (j.kk_x < 1 || j.kk_y < 1 || j.W > 512 || j.H > 32767 || 4ll * R * j.H * j.tpitch >= (1ll << 32))
             L.mfma_ok = false;
@@ -946,13 +947,34 @@ int check_knob(int knob, int value) {
       if (value < 0 || value > 512 || value % 64)
         return set_error(PS_ERR_INVALID_ARG, "threads per block must be 0 or a multiple of 64 up to 512");
       return PS_OK;
+    case PS_TUNE_FP32_COL_ROWS:
+      if (value != 0 && value != 8 && value != 16)
+        return set_error(PS_ERR_INVALID_ARG, "column-pass rows per lane must be 0, 8 or 16");
+      return PS_OK;
     default:
       return set_error(PS_ERR_INVALID_ARG, "unknown tuning knob " + std::to_string(knob));
   }
 }
-void set_knob(WidthClass &c, int knob, int value) {
+// the column-pass classes a row-pass class's knob reaches: those of its
+// level with its width class (64-column strips)
+std::vector<WidthClass *> col_classes_of(ps_plan *plan, int cls) {
+  std::vector<WidthClass *> out;
+  int k = 0;
+  for (auto *lv : {&plan->up, &plan->down})
+    for (auto &L : *lv)
+      for (auto &c : L.classes) {
+        if (k++ != cls) continue;
+        const int xt = (c.max_w + 63) / 64;
+        for (auto &cc : L.col_classes)
+          if ((cc.max_w + 63) / 64 == xt) out.push_back(&cc);
+      }
+  return out;
+}
+void set_knob(ps_plan *plan, int cls, WidthClass &c, int knob, int value) {
   if (knob == PS_TUNE_MFMA_ROWS) c.mfma_rows = value;
-  else c.row_threads = value;
+  else if (knob == PS_TUNE_FP32_ROW_THREADS) c.row_threads = value;
+  else
+    for (WidthClass *cc : col_classes_of(plan, cls)) cc->col_rows = value;
 }
 void drop_graph(ps_plan *plan) {
   if (plan->gexec) {
@@ -968,7 +990,8 @@ extern "C" int ps_plan_tune(ps_plan *plan, int knob, int value) {
   if (st != PS_OK) return st;
   drop_graph(plan);
   if (knob == PS_TUNE_MFMA_ROWS) plan->mfma_rows = value;
-  for (WidthClass *c : tune_classes(plan)) set_knob(*c, knob, knob == PS_TUNE_MFMA_ROWS ? 0 : value);
+  std::vector<WidthClass *> v = tune_classes(plan);
+  for (size_t k = 0; k < v.size(); ++k) set_knob(plan, (int)k, *v[k], knob, knob == PS_TUNE_MFMA_ROWS ? 0 : value);
   return PS_OK;
 }
 
@@ -979,7 +1002,7 @@ extern "C" int ps_plan_tune_class(ps_plan *plan, int knob, int cls, int value) {
   std::vector<WidthClass *> v = tune_classes(plan);
   if (cls < 0 || cls >= (int)v.size()) return set_error(PS_ERR_INVALID_ARG, "no launch class " + std::to_string(cls));
   drop_graph(plan);
-  set_knob(*v[cls], knob, value);
+  set_knob(plan, cls, *v[cls], knob, value);
   return PS_OK;
 }
 
@@ -1108,7 +1131,8 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
         e = launch_colpass_mfma(b, jobs + c.start, c.count, pl->frags, pl->shifts, pl->R, col_terms, terms != 3,
                                 c.epi_hi, c.max_w, terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], c.leaf, s);
       else
-        e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s);
+        e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s,
+                           c.col_rows);
       profile_end(pp, ev2, s);
     }
     // the column pass class also times the eigenbasis stages
@@ -1223,7 +1247,7 @@ static hipError_t sweep_subset(ps_plan *pl, const Bases &b, ps_map *map_dev, hip
       const WidthClass &c = L.col_classes[k];
       const long ev2 = profile_begin(pp, 3, s);
       e = launch_colpass(b, jobs + ls.col[k].start, ls.col[k].count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w,
-                         L.col_lds, s);
+                         L.col_lds, s, c.col_rows);
       profile_end(pp, ev2, s);
     }
     if (e == hipSuccess && ls.od_col.count > 0) {
@@ -1456,7 +1480,7 @@ int message_prepare(const ps_model *model, int height, int width, const double o
   ms.od = mp.od != 0;
   ms.od_cells = mp.od ? (long long)R * mp.th * mp.tw : 0;
   ms.lds = rowpass_lds_bytes(R, width, mj.kx);
-  ms.col_lds = mp.od ? 0 : colpass_lds_bytes(mp.ky);
+  ms.col_lds = mp.od ? 0 : colpass_lds_bytes(mp.ky, 16);
   if (ms.lds > kLdsLimit || ms.col_lds > kLdsLimit)
     return set_error(PS_ERR_UNSUPPORTED, "message needs more LDS than one workgroup has");
   mj.src_kind = kSrcUnary;  // a plain grid, read through the (empty) mask path

@@ -385,34 +385,36 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
 
 // -------------------------------------------------------------- k_colpass
 // One block per (message, slice r, strip of 64 columns); it walks down the
-// strip in tiles of 64 output rows.  Each tile stages the source rows it needs
-// (64 + padded taps, zero outside the grid, inverse shift folded into the
-// origin) in LDS; each of the four waves then computes 16 rows, one column per
-// lane, with a 16-row register ring fed from LDS.  Taps run in chunks of 8
-// (the tap pool is zero padded), so the ring indices are static.
+// strip in tiles of 4 ROWS output rows.  Each tile stages the source rows it
+// needs (4 ROWS + padded taps, zero outside the grid, inverse shift folded
+// into the origin) in LDS; each of the four waves then computes ROWS rows,
+// one column per lane, with a ROWS-row register ring fed from LDS.  Taps run
+// in chunks of 8 (the tap pool is zero padded), so the ring indices are
+// static.  ROWS = 8 (the default: at most 79 VGPRs, 6 waves per SIMD) or 16
+// (twice the filter work per LDS read, at most 128 VGPRs and 4 waves); the
+// outputs are the same bit for bit.
 
 constexpr int kColTileX = 64;
-constexpr int kColRows = 16;
 constexpr int kColWaves = 4;
-constexpr int kColTileY = kColRows * kColWaves;
+constexpr int kColRowsDefault = 8;
 
 __host__ __device__ __forceinline__ int col_padded_taps(int ky) { return (2 * ky + 1 + 3) & ~3; }
-__host__ __device__ __forceinline__ int col_tile_rows(int ky) { return kColTileY + col_padded_taps(ky); }
+__host__ __device__ __forceinline__ int col_tile_rows(int ky, int rows) { return kColWaves * rows + col_padded_taps(ky); }
 // rows moved per thread and pass when a tile hands its halo to the next one
 constexpr int kColMove = 24;
 
-// N (8 or 4) taps of the y filter starting at tap t; PH is the ring phase (t % 16)
-template <int PH, int N>
-__device__ __forceinline__ void col_chunk(float (&acc)[kColRows], float (&buf)[kColRows], const float *fy,
-                                          const float *tw, int t) {
+// N (8 or 4) taps of the y filter starting at tap t; PH is the ring phase (t % ROWS)
+template <int PH, int N, int ROWS>
+__device__ __forceinline__ void col_chunk(float (&acc)[ROWS], float (&buf)[ROWS], const float *fy, const float *tw,
+                                          int t) {
   float f[N];
 #pragma unroll
   for (int u = 0; u < N; ++u) f[u] = fy[t + u];
 #pragma unroll
   for (int u = 0; u < N; ++u) {
 #pragma unroll
-    for (int i = 0; i < kColRows; ++i) acc[i] = fmaf(buf[(i + u + PH) & (kColRows - 1)], f[u], acc[i]);
-    buf[(u + PH) & (kColRows - 1)] = tw[(t + u + kColRows) * kColTileX];
+    for (int i = 0; i < ROWS; ++i) acc[i] = fmaf(buf[(i + u + PH) & (ROWS - 1)], f[u], acc[i]);
+    buf[(u + PH) & (ROWS - 1)] = tw[(t + u + ROWS) * kColTileX];
   }
 }
 
@@ -421,9 +423,11 @@ __device__ __forceinline__ void col_chunk(float (&acc)[kColRows], float (&buf)[k
 // epilogue operands first, then the next tile's rows (always loaded, clamped
 // into the grid, zeroed when they reach LDS), and the epilogue waits for its
 // operands only.
-template <int EPI>
+template <int EPI, int ROWS>
 __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, const float *__restrict__ taps,
                                               const int *__restrict__ shifts, int R, int r, int xt, float *tile) {
+  constexpr int kColRows = ROWS;
+  constexpr int kColTileY = kColRows * kColWaves;
   constexpr bool kNeedU = Epi<EPI>::kNeedU;
   constexpr bool kNeedB = Epi<EPI>::kNeedB;
   const int H = J.H, W = J.W;
@@ -441,7 +445,7 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   const float *Tr = cptr(b, J.tmp) + (long long)r * H * tp;
   const int ky = J.ky;
   const int Lp = col_padded_taps(ky);
-  const int nrows = col_tile_rows(ky);
+  const int nrows = col_tile_rows(ky, ROWS);
   const float *fy = taps + J.taps_y;
   const float M = key2f(b.max_keys[J.src_key]);
   float *dst = mptr(b, J.dst) + r * HW;
@@ -508,19 +512,24 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
       a[i] = 0.0f;
       buf[i] = tw[i * kColTileX];
     }
-    // taps in 16s (the ring's period), then a tail of 4, 8 or 12 (the tap
-    // count is padded to a multiple of 4 with zero taps)
+    // taps in runs of the ring's period, then a tail (the tap count is
+    // padded to a multiple of 4 with zero taps)
     int t0 = 0;
-    for (; t0 + 16 <= Lp; t0 += 16) {
-      col_chunk<0, 8>(a, buf, fy, tw, t0);
-      col_chunk<8, 8>(a, buf, fy, tw, t0 + 8);
-    }
-    const int rem = Lp - t0;
-    if (rem >= 8) {
-      col_chunk<0, 8>(a, buf, fy, tw, t0);
-      if (rem == 12) col_chunk<8, 4>(a, buf, fy, tw, t0 + 8);
-    } else if (rem == 4) {
-      col_chunk<0, 4>(a, buf, fy, tw, t0);
+    if constexpr (ROWS == 16) {
+      for (; t0 + 16 <= Lp; t0 += 16) {
+        col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
+        col_chunk<8, 8, ROWS>(a, buf, fy, tw, t0 + 8);
+      }
+      const int rem = Lp - t0;
+      if (rem >= 8) {
+        col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
+        if (rem == 12) col_chunk<8, 4, ROWS>(a, buf, fy, tw, t0 + 8);
+      } else if (rem == 4) {
+        col_chunk<0, 4, ROWS>(a, buf, fy, tw, t0);
+      }
+    } else {
+      for (; t0 + 8 <= Lp; t0 += 8) col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
+      if (Lp - t0 == 4) col_chunk<0, 4, ROWS>(a, buf, fy, tw, t0);
     }
 
     if (kNeedB) {
@@ -573,7 +582,7 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
 // epilogue fits without spilling vector registers once its stored belief is
 // loaded after the filter (2.16 -> 1.33 ms per launch on 32 300x300x24
 // grids; unbounded, at 156 VGPRs and 3 waves per SIMD, it takes 1.65 ms).
-template <int HI>
+template <int HI, int ROWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_colpass(
     Bases b, const MsgJob *__restrict__ jobs, const float *__restrict__ taps, const int *__restrict__ shifts, int R,
     int xtiles) {
@@ -588,15 +597,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (xt * kColTileX >= J.W) return;
   const int kind = epi_kind(J);
   if (HI == kEpiUpRoot || kind == kEpiUpRoot)
-    colpass_strip<kEpiUpRoot>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiUpRoot, ROWS>(b, J, taps, shifts, R, r, xt, tile);
   else if (HI == kEpiUpNode || kind == kEpiUpNode)
-    colpass_strip<kEpiUpNode>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiUpNode, ROWS>(b, J, taps, shifts, R, r, xt, tile);
   else if (HI == kEpiUpNodeU || kind == kEpiUpNodeU)
-    colpass_strip<kEpiUpNodeU>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiUpNodeU, ROWS>(b, J, taps, shifts, R, r, xt, tile);
   else if (HI == kEpiDownLeaf || kind == kEpiDownLeaf)
-    colpass_strip<kEpiDownLeaf>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiDownLeaf, ROWS>(b, J, taps, shifts, R, r, xt, tile);
   else
-    colpass_strip<kEpiDownMid>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiDownMid, ROWS>(b, J, taps, shifts, R, r, xt, tile);
 }
 
 // ----------------------------------------------------------------- k_root
@@ -824,12 +833,20 @@ hipError_t rowpass_set_lds_limit(size_t bytes) {
   return raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_any), bytes);
 }
 
+template <int ROWS>
+static hipError_t colpass_rows_set_lds_limit(size_t bytes) {
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpRoot, ROWS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNode, ROWS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNodeU, ROWS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownLeaf, ROWS>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownMid, ROWS>), bytes);
+  return e;
+}
+
+// bytes: the LDS of the 16-row tile (the larger one)
 hipError_t colpass_set_lds_limit(size_t bytes) {
-  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpRoot>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNode>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNodeU>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownLeaf>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownMid>), bytes);
+  hipError_t e = colpass_rows_set_lds_limit<8>(bytes);
+  if (e == hipSuccess) e = colpass_rows_set_lds_limit<16>(bytes);
   return e;
 }
 
@@ -848,23 +865,40 @@ hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
   return hipGetLastError();
 }
 
-size_t colpass_lds_bytes(int ky) { return sizeof(float) * (size_t)col_tile_rows(ky) * kColTileX; }
+// LDS of a column-pass tile: rows 0 = the larger (16-row) tile
+size_t colpass_lds_bytes(int ky, int rows) {
+  return sizeof(float) * (size_t)col_tile_rows(ky, rows > 0 ? rows : 16) * kColTileX;
+}
 
-// epi_hi: the highest destination kind (EpiKind) among the jobs
-hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int epi_hi, int max_w, size_t lds_bytes, hipStream_t s) {
-  if (n_jobs <= 0) return hipSuccess;
-  if (epi_hi < kEpiUpRoot || epi_hi > kEpiDownMid) return hipErrorInvalidValue;
-  const int xtiles = (max_w + kColTileX - 1) / kColTileX;
-  dim3 grid(R * xtiles, n_jobs);
+template <int ROWS>
+static void launch_colpass_rows(int epi_hi, dim3 grid, size_t lds_bytes, hipStream_t s, const Bases &b,
+                                const MsgJob *jobs, const float *taps, const int *shifts, int R, int xtiles) {
   const dim3 blk(kColWaves * 64);
   switch (epi_hi) {
-    case kEpiUpRoot: hipLaunchKernelGGL(k_colpass<kEpiUpRoot>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    case kEpiUpNode: hipLaunchKernelGGL(k_colpass<kEpiUpNode>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    case kEpiUpNodeU: hipLaunchKernelGGL(k_colpass<kEpiUpNodeU>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    case kEpiDownLeaf: hipLaunchKernelGGL(k_colpass<kEpiDownLeaf>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    default: hipLaunchKernelGGL(k_colpass<kEpiDownMid>, grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpRoot: hipLaunchKernelGGL((k_colpass<kEpiUpRoot, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpNode: hipLaunchKernelGGL((k_colpass<kEpiUpNode, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass<kEpiUpNodeU, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass<kEpiDownLeaf, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    default: hipLaunchKernelGGL((k_colpass<kEpiDownMid, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
   }
+}
+
+// epi_hi: the highest destination kind (EpiKind) among the jobs; rows:
+// output rows per lane (8 or 16; 0 = the default); lds_bytes_16: the LDS of
+// the 16-row tile for the jobs' widest y filter (the 8-row tile takes 32
+// rows of 256 bytes less)
+hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
+                          int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows) {
+  if (n_jobs <= 0) return hipSuccess;
+  if (rows == 0) rows = kColRowsDefault;
+  if (epi_hi < kEpiUpRoot || epi_hi > kEpiDownMid || (rows != 8 && rows != 16)) return hipErrorInvalidValue;
+  const int xtiles = (max_w + kColTileX - 1) / kColTileX;
+  dim3 grid(R * xtiles, n_jobs);
+  if (rows == 16)
+    launch_colpass_rows<16>(epi_hi, grid, lds_bytes_16, s, b, jobs, taps, shifts, R, xtiles);
+  else
+    launch_colpass_rows<8>(epi_hi, grid, lds_bytes_16 - sizeof(float) * kColWaves * 8 * kColTileX, s, b, jobs, taps,
+                           shifts, R, xtiles);
   return hipGetLastError();
 }
 

@@ -96,8 +96,9 @@ def test_default_rotation_grid_48(gpu):
 
 def test_launch_shape_knobs_do_not_change_results(gpu):
     # the FP32 row pass at 64..512 threads per block (a thread then covers
-    # several columns) and per-class settings of both knobs: bit for bit the
-    # default launch shapes
+    # several columns), the FP32 column pass at 8 and 16 output rows per
+    # lane, the BF16 x pass's rows per block, and per-class settings of each
+    # knob: bit for bit the default launch shapes
     import torch
     from partapp_amd._lib import PsError
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
@@ -110,6 +111,7 @@ def test_launch_shape_knobs_do_not_change_results(gpu):
     say("launch classes:", classes)
     assert len(classes) >= 4 and all(c["jobs"] > 0 for c in classes)
     for method, knob, values in (("fp32", plan.TUNE_FP32_ROW_THREADS, (64, 128, 256, 512)),
+                                 ("fp32", plan.TUNE_FP32_COL_ROWS, (8, 16)),
                                  ("bf16mix", plan.TUNE_MFMA_ROWS, (1, 3, 16))):
         plan.set_method(method)
         plan.tune(knob, 0 if method == "fp32" else 4)
@@ -131,6 +133,8 @@ def test_launch_shape_knobs_do_not_change_results(gpu):
         assert torch.equal(marg, ref[0]), method
     with pytest.raises(PsError):
         plan.tune(plan.TUNE_FP32_ROW_THREADS, 100)
+    with pytest.raises(PsError):
+        plan.tune(plan.TUNE_FP32_COL_ROWS, 12)
     with pytest.raises(PsError):
         plan.tune(plan.TUNE_MFMA_ROWS, 4, cls=len(classes))
     plan.close()

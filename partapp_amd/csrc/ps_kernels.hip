@@ -235,7 +235,8 @@ __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x
   }
 }
 
-// Row pass for a rotation count fixed at compile time: each lane owns one
+// Row pass for a rotation count fixed at compile time (24, and the
+// reference's default 48 at 173 VGPRs, 2 waves per SIMD): each lane owns one
 // column x; its R source values are loaded together (clamped addresses, then
 // a select), exponentiated, rotation-filtered in registers and written to P.
 template <int R>
@@ -829,6 +830,7 @@ hipError_t raise_lds_limit(const void *f, size_t bytes) {
 
 hipError_t rowpass_set_lds_limit(size_t bytes) {
   hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<24>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<48>), bytes);
   if (e != hipSuccess) return e;
   return raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_any), bytes);
 }
@@ -860,6 +862,8 @@ hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
                              : std::min(512, (max_w + 63) / 64 * 64);
   if (R == 24)
     hipLaunchKernelGGL(k_rowpass_r<24>, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
+  else if (R == 48)  // the reference's default grid: 2.4x the generic pass's sweep rate at 300x300
+    hipLaunchKernelGGL(k_rowpass_r<48>, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
   else
     hipLaunchKernelGGL(k_rowpass_any, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts, R);
   return hipGetLastError();

@@ -5,8 +5,8 @@ the FP32 bar (relative L1 <= 1e-5 per part, identical MAP states):
   creating the narrower plan must not lower the wider plan's LDS limit;
 * a y filter with half width >= 96 (sigma_y * scale >= 32 px), whose FP32
   column-pass tile needs more than 64 KB of LDS;
-* the reference's default rotation grid, 48 steps (ExpParam.proto:195), which
-  runs the generic FP32 row pass (its BF16 methods: tests/test_gpu_bf16.py).
+* the reference's default rotation grid, 48 steps (ExpParam.proto:195), on the
+  48-slice FP32 row pass (its BF16 methods: tests/test_gpu_bf16.py).
 """
 import sys
 

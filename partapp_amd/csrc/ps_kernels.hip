@@ -115,14 +115,16 @@ __global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__
 
 struct RowGeom {
   int Wo;    // groups of 8 outputs per row
-  int nch;   // chunks of 8 taps
+  int nch;   // chunks of 8 taps (the last one a half chunk when tail4)
   int Pw;    // padded row pitch of P, a multiple of 8
+  bool tail4;  // the taps, padded to a multiple of 4, end in a chunk of 4
 };
 __host__ __device__ __forceinline__ RowGeom row_geom(int W, int kx) {
   RowGeom g;
   g.Wo = (W + 7) >> 3;
   g.nch = (2 * kx + 1 + 7) >> 3;
   g.Pw = 8 * (g.Wo + g.nch + 1);
+  g.tail4 = ((2 * kx + 1 + 3) & 7) == 4;
   return g;
 }
 
@@ -142,10 +144,13 @@ __device__ __forceinline__ void zero_halo(float *P, int R, int W, int kx, const 
 // row y of T (row pitch tp, a multiple of 32: the outputs past W land in the
 // padding)
 // NCH chunks known at compile time: the chunk loop unrolls, so the sliding
-// window is renamed instead of copied
+// window is renamed instead of copied.  With g.tail4 the last chunk runs its
+// first 4 taps only (the others are the zero padding: an fmaf with a zero tap
+// leaves the sum as it is)
 template <int NCH>
 __device__ __forceinline__ void xfilter_rows_n(const float *P, const float *taps_x, float *T, int R, int tp,
                                                long long plane, int y, const RowGeom &g) {
+  const bool tail4 = g.tail4;
   const int n_items = R * g.Wo;
   const float inv_wo = 1.0f / (float)g.Wo;
   for (int item = threadIdx.x; item < n_items; item += blockDim.x) {
@@ -165,7 +170,7 @@ __device__ __forceinline__ void xfilter_rows_n(const float *P, const float *taps
     const float *tx = taps_x;
     asm volatile("" : "+s"(tx));
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
+    for (int c = 0; c < NCH - 1; ++c) {
       const float4 w2 = P4[2 * c + 2], w3 = P4[2 * c + 3];
       const float win[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
                              w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
@@ -178,6 +183,28 @@ __device__ __forceinline__ void xfilter_rows_n(const float *P, const float *taps
         for (int i = 0; i < 8; ++i) a[i] = fmaf(win[i + u], f[u], a[i]);
       w0 = w2;
       w1 = w3;
+    }
+    {  // the last chunk: 4 or 8 taps
+      constexpr int c = NCH - 1;
+      const float4 w2 = P4[2 * c + 2];
+      float f[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f[u] = tx[8 * c + u];
+      if (tail4) {
+        const float win[12] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w, w2.x, w2.y, w2.z, w2.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int i = 0; i < 8; ++i) a[i] = fmaf(win[i + u], f[u], a[i]);
+      } else {
+        const float4 w3 = P4[2 * c + 3];
+        const float win[16] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                               w2.x, w2.y, w2.z, w2.w, w3.x, w3.y, w3.z, w3.w};
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int i = 0; i < 8; ++i) a[i] = fmaf(win[i + u], f[u], a[i]);
+      }
     }
     float *trow = T + ro * plane + (long long)y * tp;
     const int x0 = 8 * q;

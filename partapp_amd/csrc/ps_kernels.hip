@@ -113,6 +113,12 @@ __global__ __launch_bounds__(256) void k_max(Bases b, const MaxJob *__restrict__
 // P[R][Pw]: the rotation-filtered rows with kx zeros on the left and a zero
 // tail, Pw a multiple of 4, read by the x filter.
 
+// slice groups the fixed-R row pass stages in LDS one after another: 48
+// slices in two halves (36 KB instead of 72 KB of LDS per 300-wide row, 160
+// VGPRs: 20.9 instead of 27.6 ms per 32 x 300x300x48 sweep), 24 in one (as
+// two halves 9.7 instead of 8.8 ms per 32 x 300x300x24 sweep)
+__host__ __device__ constexpr int row_groups(int R) { return R >= 48 ? 2 : 1; }
+
 struct RowGeom {
   int Wo;    // groups of 8 outputs per row
   int nch;   // chunks of 8 taps (the last one a half chunk when tail4)
@@ -262,10 +268,73 @@ __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x
   }
 }
 
+// One column x of the fixed-R row pass for slice group G: the R source
+// values are loaded together (clamped addresses, then a select),
+// exponentiated and rotation-filtered in registers; the group's output slices
+// G * RG .. (G + 1) * RG - 1 are written to P.
+template <int R, int G>
+__device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__restrict__ taps,
+                                               const int *__restrict__ shifts, const float *src, float negM,
+                                               float e_fill, float *P, int y, const RowGeom &g, const int x) {
+  constexpr int RG = R / row_groups(R);
+  const int H = J.H, W = J.W;
+  const int kx = J.kx;
+  // per OUTPUT slice: the source slice is ro - delta, read at (y + oyo, x + oxo);
+  // slices without a source carry an offset that fails the bounds test
+  const int *oxo = shifts + J.shifts + 4 * R;
+  const int *oyo = oxo + R;
+  const float *fr = taps + J.taps_r;
+  const int delta = J.delta;
+  const bool zero_src = J.src_kind == kSrcZero;
+  const UnaryMask mk = J.src_mask;
+  const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
+  const int HW32 = H * W;  // R * H * W < 2^31 is checked when the plan is built
+
+  // forward warp (objectdetect_findrot.cpp:342-359); the loads of all
+  // slices are issued before any of them is used.  off < 0: no source
+  // pixel, the warp's LOG_ZERO fill.
+  int off[R];
+#pragma unroll
+  for (int ro = 0; ro < R; ++ro) {
+    const int yy = y + oyo[ro];
+    const int xx = x + oxo[ro];
+    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    off[ro] = ok ? (ro - delta) * HW32 + yy * W + xx : -1;
+  }
+  float e[R];
+  if (zero_src) {
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) e[ro] = 0.0f;
+  } else {
+#pragma unroll
+    for (int ro = 0; ro < R; ++ro) e[ro] = src[off[ro] < 0 ? 0 : off[ro]];
+  }
+  // masks, then exp(b - M) (:362-365) as 2^((b - M) log2 e) on the hardware
+  // exp2 (values below 2^-126 flush to zero, where the reference's expf
+  // returns subnormals)
+#pragma unroll
+  for (int ro = 0; ro < R; ++ro) {
+    float vv = e[ro];
+    if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
+    e[ro] = off[ro] < 0 ? e_fill : __builtin_amdgcn_exp2f((vv + negM) * kLog2eF);
+  }
+  float o[RG];
+  if (J.rot_mode == kRotFilter) {
+    RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1, G * RG, (G + 1) * RG>::run(J.rot_len, e, o, fr);
+  } else if (J.rot_mode == kRotCopy) {
+#pragma unroll
+    for (int k = 0; k < RG; ++k) o[k] = e[G * RG + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < RG; ++k) o[k] = 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < RG; ++k) P[k * g.Pw + kx + x] = o[k];
+}
+
 // Row pass for a rotation count fixed at compile time (24, and the
-// reference's default 48 at 173 VGPRs, 2 waves per SIMD): each lane owns one
-// column x; its R source values are loaded together (clamped addresses, then
-// a select), exponentiated, rotation-filtered in registers and written to P.
+// reference's default 48): lane t owns the columns x = t (mod blockDim.x);
+// the slice groups go through P and the x filter one after another.
 template <int R>
 __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__restrict__ jobs,
                                                    const float *__restrict__ taps, const int *__restrict__ shifts) {
@@ -280,67 +349,26 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
   if (y >= H) return;
   const int kx = J.kx;
   const RowGeom g = row_geom(W, kx);
-  zero_halo(P, R, W, kx, g);
+  constexpr int RG = R / row_groups(R);
+  zero_halo(P, RG, W, kx, g);
 
   const float M = key2f(b.max_keys[J.src_key]);
   const float negM = -M;
   const float e_fill = expf(kLogZero + negM);  // exp(LOG_ZERO - M): the warp's fill value
   const float *src = cptr(b, J.src);
-  // per OUTPUT slice: the source slice is ro - delta, read at (y + oyo, x + oxo);
-  // slices without a source carry an offset that fails the bounds test
-  const int *oxo = shifts + J.shifts + 4 * R;
-  const int *oyo = oxo + R;
-  const float *fr = taps + J.taps_r;
-  const int delta = J.delta;
-  const bool zero_src = J.src_kind == kSrcZero;
-  const UnaryMask mk = J.src_mask;
-  const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
-  const int HW32 = H * W;  // R * H * W < 2^31 is checked when the plan is built
-
-  for (int x = threadIdx.x; x < W; x += blockDim.x) {
-    // forward warp (objectdetect_findrot.cpp:342-359); the loads of all
-    // slices are issued before any of them is used.  off < 0: no source
-    // pixel, the warp's LOG_ZERO fill.
-    int off[R];
-#pragma unroll
-    for (int ro = 0; ro < R; ++ro) {
-      const int yy = y + oyo[ro];
-      const int xx = x + oxo[ro];
-      const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-      off[ro] = ok ? (ro - delta) * HW32 + yy * W + xx : -1;
-    }
-    float e[R];
-    if (zero_src) {
-#pragma unroll
-      for (int ro = 0; ro < R; ++ro) e[ro] = 0.0f;
-    } else {
-#pragma unroll
-      for (int ro = 0; ro < R; ++ro) e[ro] = src[off[ro] < 0 ? 0 : off[ro]];
-    }
-    // masks, then exp(b - M) (:362-365) as 2^((b - M) log2 e) on the hardware
-    // exp2 (values below 2^-126 flush to zero, where the reference's expf
-    // returns subnormals)
-#pragma unroll
-    for (int ro = 0; ro < R; ++ro) {
-      float vv = e[ro];
-      if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
-      e[ro] = off[ro] < 0 ? e_fill : __builtin_amdgcn_exp2f((vv + negM) * kLog2eF);
-    }
-    float o[R];
-    if (J.rot_mode == kRotFilter) {
-      RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1>::run(J.rot_len, e, o, fr);
-    } else if (J.rot_mode == kRotCopy) {
-#pragma unroll
-      for (int ro = 0; ro < R; ++ro) o[ro] = e[ro];
-    } else {
-#pragma unroll
-      for (int ro = 0; ro < R; ++ro) o[ro] = 0.0f;
-    }
-#pragma unroll
-    for (int ro = 0; ro < R; ++ro) P[ro * g.Pw + kx + x] = o[ro];
-  }
+  float *T = mptr(b, J.tmp);
+  const long long plane = (long long)H * J.tpitch;
+  for (int x = threadIdx.x; x < W; x += blockDim.x)
+    rowpass_column<R, 0>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
   __syncthreads();
-  xfilter_rows(P, taps + J.taps_x, mptr(b, J.tmp), R, J.tpitch, (long long)H * J.tpitch, y, g);
+  xfilter_rows(P, taps + J.taps_x, T, RG, J.tpitch, plane, y, g);
+  if constexpr (row_groups(R) > 1) {
+    __syncthreads();  // every item of the first group is filtered
+    for (int x = threadIdx.x; x < W; x += blockDim.x)
+      rowpass_column<R, 1>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+    __syncthreads();
+    xfilter_rows(P, taps + J.taps_x, T + RG * plane, RG, J.tpitch, plane, y, g);
+  }
 }
 
 // Row pass for any rotation count: the exponentiated column goes through an
@@ -827,12 +855,13 @@ hipError_t launch_max(const Bases &b, const MaxJob *jobs, int n_jobs, int R, int
   return hipGetLastError();
 }
 
-static bool rowpass_fixed(int R) { return R == 24; }
+static bool rowpass_fixed(int R) { return R == 24 || R == 48; }
 
 size_t rowpass_lds_bytes(int R, int W, int kx) {
   RowGeom g = row_geom(W, kx);
-  size_t p = (size_t)R * g.Pw;
-  if (!rowpass_fixed(R)) p += (size_t)R * ((W + 3) & ~3);  // the E buffer of the generic kernel
+  // the fixed-R kernels stage one slice group at a time; the generic one all
+  // slices and its E buffer
+  size_t p = rowpass_fixed(R) ? (size_t)(R / row_groups(R)) * g.Pw : (size_t)R * g.Pw + (size_t)R * ((W + 3) & ~3);
   return sizeof(float) * p;
 }
 

@@ -12,32 +12,33 @@ namespace ps {
 // Circular rotation filter with LR taps on a register column
 // (objectdetect_findrot.cpp:377-420, multi_array_filter.hpp:115-155):
 // out[r] = sum_j in[(r - n + j) mod R] f[j], taps in ascending order.
-template <int R, int LR>
-__device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], float (&o)[R], const float *fr) {
+// outputs LO .. HI - 1 of the circular rotation filter into o[0 .. HI - LO)
+template <int R, int LR, int LO, int HI>
+__device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], float (&o)[HI - LO], const float *fr) {
   constexpr int n = (LR - 1) / 2;
   float f[LR];
 #pragma unroll
   for (int j = 0; j < LR; ++j) f[j] = fr[j];
 #pragma unroll
-  for (int ro = 0; ro < R; ++ro) {
+  for (int ro = LO; ro < HI; ++ro) {
     float acc = 0.0f;
 #pragma unroll
     for (int j = 0; j < LR; ++j) acc = fmaf(e[((ro - n + j) % R + R) % R], f[j], acc);
-    o[ro] = acc;
+    o[ro - LO] = acc;
   }
 }
-template <int R, int LR>
+template <int R, int LR, int LO, int HI>
 struct RotDispatch {
-  static __device__ __forceinline__ void run(int lr, const float (&e)[R], float (&o)[R], const float *fr) {
-    if (lr == LR) rot_filter_fixed<R, LR>(e, o, fr);
-    else RotDispatch<R, LR - 2>::run(lr, e, o, fr);
+  static __device__ __forceinline__ void run(int lr, const float (&e)[R], float (&o)[HI - LO], const float *fr) {
+    if (lr == LR) rot_filter_fixed<R, LR, LO, HI>(e, o, fr);
+    else RotDispatch<R, LR - 2, LO, HI>::run(lr, e, o, fr);
   }
 };
-template <int R>
-struct RotDispatch<R, -1> {
-  static __device__ __forceinline__ void run(int, const float (&)[R], float (&o)[R], const float *) {
+template <int R, int LO, int HI>
+struct RotDispatch<R, -1, LO, HI> {
+  static __device__ __forceinline__ void run(int, const float (&)[R], float (&o)[HI - LO], const float *) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) o[r] = 0.0f;
+    for (int r = 0; r < HI - LO; ++r) o[r] = 0.0f;
   }
 };
 

@@ -433,6 +433,9 @@ struct Builder {
       const int oy = has ? std::max(-32767, std::min(32767, mp.fwd_oy[rs])) : -32768;
       shifts.push_back((int)(((unsigned)oy << 16) | ((unsigned)ox & 0xffffu)));
     }
+    // the source slice of each output slice clamped into [0, R) (an address
+    // inside the grid for a slice without a source, whose loads are discarded)
+    for (int ro = 0; ro < R; ++ro) shifts.push_back(std::max(0, std::min(R - 1, ro - mp.delta)));
     return PS_OK;
   }
 };

@@ -269,9 +269,9 @@ __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x
 }
 
 // One column x of the fixed-R row pass for slice group G: the R source
-// values are loaded together (clamped addresses, then a select),
-// exponentiated and rotation-filtered in registers; the group's output slices
-// G * RG .. (G + 1) * RG - 1 are written to P.
+// values are loaded together, exponentiated and rotation-filtered in
+// registers; the group's output slices G * RG .. (G + 1) * RG - 1 are written
+// to P.
 template <int R, int G>
 __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__restrict__ taps,
                                                const int *__restrict__ shifts, const float *src, float negM,
@@ -279,57 +279,69 @@ __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__r
   constexpr int RG = R / row_groups(R);
   const int H = J.H, W = J.W;
   const int kx = J.kx;
-  // per OUTPUT slice: the source slice is ro - delta, read at (y + oyo, x + oxo);
-  // slices without a source carry an offset that fails the bounds test
-  const int *oxo = shifts + J.shifts + 4 * R;
-  const int *oyo = oxo + R;
   const float *fr = taps + J.taps_r;
   const int delta = J.delta;
   const bool zero_src = J.src_kind == kSrcZero;
   const UnaryMask mk = J.src_mask;
   const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
-  const int HW32 = H * W;  // R * H * W < 2^31 is checked when the plan is built
+  int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
+  // per OUTPUT slice: the source slice is ro - delta (clamped into [0, R) at
+  // so + 3 R), read at (y + oyo, x + oxo) with oxo at so, oyo at so + R;
+  // slices without a source carry a row offset that fails the bounds test.
+  // The offsets are read through the scalar cache twice (an empty asm keeps
+  // the reads apart) rather than held in scalar registers from the loads to
+  // their use.
+  int so = J.shifts + 4 * R;
+  const int ys = __builtin_amdgcn_readfirstlane(y);  // uniform: the row offsets on the scalar unit
 
-  // forward warp (objectdetect_findrot.cpp:342-359); the loads of all
-  // slices are issued before any of them is used.  off < 0: no source
-  // pixel, the warp's LOG_ZERO fill.
-  int off[R];
+  // forward warp (objectdetect_findrot.cpp:342-359): every load unconditional
+  // (a slice without a source reads its clamped slice, a column outside the
+  // grid its nearest column), all R of them issued before any is used; the
+  // row address on the scalar unit, the column a byte offset from it
+  float v[R];
+  asm volatile("" : "+s"(so), "+s"(HW));
+  const int x4 = x * 4, xmax4 = (W - 1) * 4;
 #pragma unroll
   for (int ro = 0; ro < R; ++ro) {
-    const int yy = y + oyo[ro];
-    const int xx = x + oxo[ro];
-    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-    off[ro] = ok ? (ro - delta) * HW32 + yy * W + xx : -1;
-  }
-  float e[R];
-  if (zero_src) {
-#pragma unroll
-    for (int ro = 0; ro < R; ++ro) e[ro] = 0.0f;
-  } else {
-#pragma unroll
-    for (int ro = 0; ro < R; ++ro) e[ro] = src[off[ro] < 0 ? 0 : off[ro]];
+    const int yy = min(max(ys + shifts[so + R + ro], 0), H - 1);
+    const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * W) * 4u;
+    const int xx4 = min(max(x4 + shifts[so + ro] * 4, 0), xmax4);
+    v[ro] = *reinterpret_cast<const float *>(srow + (unsigned)xx4);
   }
   // masks, then exp(b - M) (:362-365) as 2^((b - M) log2 e) on the hardware
   // exp2 (values below 2^-126 flush to zero, where the reference's expf
-  // returns subnormals)
+  // returns subnormals).  A source pixel outside the grid (the warp's
+  // LOG_ZERO fill) and a masked unary state (a masked rotation, or a column
+  // in the masked strips) both take e_fill = exp(LOG_ZERO - M): a masked
+  // rotation fails its row's test, the strips narrow the columns' range.
+  asm volatile("" : "+s"(so));
+  const int lo = use_mask ? mk.strip : 0;
+  const unsigned span = (unsigned)max(W - 2 * lo, 0);  // columns lo .. W - lo - 1 are read
+  const unsigned long long rot_bits = use_mask ? mk.rot_bits : 0ull;
+  float e[R];
 #pragma unroll
   for (int ro = 0; ro < R; ++ro) {
-    float vv = e[ro];
-    if (use_mask && masked(mk, ro - delta, x + oxo[ro], W)) vv = kLogZero;
-    e[ro] = off[ro] < 0 ? e_fill : __builtin_amdgcn_exp2f((vv + negM) * kLog2eF);
+    const int rs = ro - delta;
+    const bool row_ok = (unsigned)(ys + shifts[so + R + ro]) < (unsigned)H &&
+                        !((unsigned)rs < (unsigned)R && ((rot_bits >> (rs & 63)) & 1ull));
+    // a leaf without a detector has the zero log posterior (b = 0); the
+    // exponential is computed for every lane (an empty asm keeps it out of a
+    // branch), then selected
+    float ev = __builtin_amdgcn_exp2f(((zero_src ? 0.0f : v[ro]) + negM) * kLog2eF);
+    asm volatile("" : "+v"(ev));
+    const bool ok = row_ok && (unsigned)(x + shifts[so + ro] - lo) < span;
+    e[ro] = ok ? ev : e_fill;
   }
-  float o[RG];
+  float *out = P + kx + x;
   if (J.rot_mode == kRotFilter) {
-    RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1, G * RG, (G + 1) * RG>::run(J.rot_len, e, o, fr);
+    RotDispatch<R, (R % 2 == 1) ? R - 2 : R - 1, G * RG, (G + 1) * RG>::run(J.rot_len, e, fr, out, g.Pw);
   } else if (J.rot_mode == kRotCopy) {
 #pragma unroll
-    for (int k = 0; k < RG; ++k) o[k] = e[G * RG + k];
+    for (int k = 0; k < RG; ++k) out[k * g.Pw] = e[G * RG + k];
   } else {
 #pragma unroll
-    for (int k = 0; k < RG; ++k) o[k] = 0.0f;
+    for (int k = 0; k < RG; ++k) out[k * g.Pw] = 0.0f;
   }
-#pragma unroll
-  for (int k = 0; k < RG; ++k) P[k * g.Pw + kx + x] = o[k];
 }
 
 // Row pass for a rotation count fixed at compile time (24, and the
@@ -354,7 +366,8 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
 
   const float M = key2f(b.max_keys[J.src_key]);
   const float negM = -M;
-  const float e_fill = expf(kLogZero + negM);  // exp(LOG_ZERO - M): the warp's fill value
+  // exp(LOG_ZERO - M): the warp's fill value, on the path of every other value
+  const float e_fill = __builtin_amdgcn_exp2f((kLogZero + negM) * kLog2eF);
   const float *src = cptr(b, J.src);
   float *T = mptr(b, J.tmp);
   const long long plane = (long long)H * J.tpitch;

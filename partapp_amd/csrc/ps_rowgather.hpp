@@ -11,10 +11,10 @@ namespace ps {
 
 // Circular rotation filter with LR taps on a register column
 // (objectdetect_findrot.cpp:377-420, multi_array_filter.hpp:115-155):
-// out[r] = sum_j in[(r - n + j) mod R] f[j], taps in ascending order.
-// outputs LO .. HI - 1 of the circular rotation filter into o[0 .. HI - LO)
+// out[r] = sum_j in[(r - n + j) mod R] f[j], taps in ascending order.  Outputs
+// LO .. HI - 1 are stored as they are formed, output r at out[(r - LO) pitch].
 template <int R, int LR, int LO, int HI>
-__device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], float (&o)[HI - LO], const float *fr) {
+__device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], const float *fr, float *out, int pitch) {
   constexpr int n = (LR - 1) / 2;
   float f[LR];
 #pragma unroll
@@ -24,21 +24,21 @@ __device__ __forceinline__ void rot_filter_fixed(const float (&e)[R], float (&o)
     float acc = 0.0f;
 #pragma unroll
     for (int j = 0; j < LR; ++j) acc = fmaf(e[((ro - n + j) % R + R) % R], f[j], acc);
-    o[ro - LO] = acc;
+    out[(ro - LO) * pitch] = acc;
   }
 }
 template <int R, int LR, int LO, int HI>
 struct RotDispatch {
-  static __device__ __forceinline__ void run(int lr, const float (&e)[R], float (&o)[HI - LO], const float *fr) {
-    if (lr == LR) rot_filter_fixed<R, LR, LO, HI>(e, o, fr);
-    else RotDispatch<R, LR - 2, LO, HI>::run(lr, e, o, fr);
+  static __device__ __forceinline__ void run(int lr, const float (&e)[R], const float *fr, float *out, int pitch) {
+    if (lr == LR) rot_filter_fixed<R, LR, LO, HI>(e, fr, out, pitch);
+    else RotDispatch<R, LR - 2, LO, HI>::run(lr, e, fr, out, pitch);
   }
 };
 template <int R, int LO, int HI>
 struct RotDispatch<R, -1, LO, HI> {
-  static __device__ __forceinline__ void run(int, const float (&)[R], float (&o)[HI - LO], const float *) {
+  static __device__ __forceinline__ void run(int, const float (&)[R], const float *, float *out, int pitch) {
 #pragma unroll
-    for (int r = 0; r < HI - LO; ++r) o[r] = 0.0f;
+    for (int r = 0; r < HI - LO; ++r) out[r * pitch] = 0.0f;
   }
 };
 

@@ -38,7 +38,8 @@ hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
                           int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows = 0);
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
-hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, hipStream_t s);
+hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, bool lse64,
+                       hipStream_t s);
 size_t rowpass_mfma_lds_bytes(int R, int W, int kx, int KK, int terms);
 size_t colpass_mfma_lds_bytes(int KK, int terms);
 size_t colpass_f16_lds_bytes(int KK, int H);
@@ -1151,7 +1152,9 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
   }
   if (e == hipSuccess) {
     ev = profile_begin(pp, 4, s);
-    e = launch_root(b, pl->roots, pl->n_roots, pl->root_children, pl->R, pl->max_hw, s);
+    // the BF16 methods accumulate the root's log-sum-exp in FP64 (shifted),
+    // the FP32 sweep in FP32 as the reference does
+    e = launch_root(b, pl->roots, pl->n_roots, pl->root_children, pl->R, pl->max_hw, terms != 0, s);
     profile_end(pp, ev, s);
   }
   for (const Level &L : pl->down) {
@@ -1264,7 +1267,7 @@ static hipError_t sweep_subset(ps_plan *pl, const Bases &b, ps_map *map_dev, hip
   if (e == hipSuccess) {
     ev = profile_begin(pp, 4, s);
     e = launch_root(b, reinterpret_cast<const RootJob *>(pl->ctables + o_r), (int)cr.size(), pl->root_children,
-                    pl->R, pl->max_hw, s);
+                    pl->R, pl->max_hw, false, s);
     profile_end(pp, ev, s);
   }
   for (size_t l = 0; l < pl->down.size() && e == hipSuccess; ++l) level(pl->down[l], down[l]);

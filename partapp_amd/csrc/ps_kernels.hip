@@ -681,7 +681,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // KC: the number of root children when it is at most 8 (fully unrolled),
 // else kMaxRootChildren with the count read at run time.
-template <int KC>
+// LSE64: the root posterior's log-sum-exp over rotations accumulated in FP64
+// with a running maximum shift (the BF16 methods); otherwise in FP32 with
+// no shift, as the reference sums expf (:713-726: a belief above ~88.7
+// overflows to +inf there and here)
+template <int KC, bool LSE64>
 __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, int R, int pix0) {
   const int H = J.H, W = J.W;
   const int HW = H * W;
@@ -700,6 +704,8 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
   unsigned second = 0u;  // second largest root belief (key), for the MAP certificate
   float sum = 0.0f;
   bool first = true;
+  double sum_d = 0.0;           // LSE64: sum of exp(belief - mref)
+  float mref = -INFINITY;       //        the largest belief so far
 
   // the loads of slice r + 1 are issued before slice r is processed (its
   // stores could alias them as far as the compiler knows)
@@ -747,17 +753,26 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
         }
       }
     }
-    // root posterior over the valid rotations (:713-726), no max shift
+    // root posterior over the valid rotations (:713-726)
     if ((J.valid_rot >> r) & 1ull) {
-      const float ev = expf(bel);
-      sum = first ? ev : sum + ev;
-      first = false;
+      if (LSE64) {
+        if (bel > mref) {
+          sum_d *= (double)__builtin_amdgcn_exp2f((mref - bel) * kLog2eF);
+          mref = bel;
+        }
+        sum_d += (double)__builtin_amdgcn_exp2f((bel - mref) * kLog2eF);
+      } else {
+        const float ev = expf(bel);
+        sum = first ? ev : sum + ev;
+        first = false;
+      }
     }
 #pragma unroll
     for (int k = 0; k < KC; ++k) m[k] = mn[k];
     u = un_next;
   }
-  if (ok && J.root_post.buf >= 0 && b.root) mptr(b, J.root_post)[pix] = log0(sum);
+  if (ok && J.root_post.buf >= 0 && b.root)
+    mptr(b, J.root_post)[pix] = LSE64 ? (sum_d > 0.0 ? log0((float)sum_d) + mref : kLogZero) : log0(sum);
 
   const int lane = threadIdx.x & 63;
   if (b.top2_keys) top2_flush(b.top2_keys + J.map_slot, (unsigned)(amax >> 32), second, lane);
@@ -775,7 +790,7 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
 // one instantiation per child count, chosen on the host: the variant for a
 // small tree then holds only its own K message registers (46 VGPRs for K = 5,
 // 8 waves per SIMD), not those of the widest tree
-template <int KC>
+template <int KC, bool LSE64>
 __global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict__ jobs, int R) {
   unsigned bx, by;
   xcd_block(bx, by);
@@ -783,7 +798,7 @@ __global__ __launch_bounds__(256) void k_root(Bases b, const RootJob *__restrict
   if (skip_inst(b, J.inst)) return;
   const int pix0 = (int)bx * blockDim.x;
   if (pix0 >= J.H * J.W) return;
-  root_pixels<KC>(b, J, R, pix0);
+  root_pixels<KC, LSE64>(b, J, R, pix0);
 }
 
 // ------------------------------------------------------------ k_map_decode
@@ -977,22 +992,31 @@ hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const 
 
 // k: the root's child count, the same for every job (one tree per plan); a
 // root without children and counts above 8 take the variant that reads J.K
-hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, hipStream_t s) {
+template <bool LSE64>
+static void launch_root_k(const Bases &b, const RootJob *jobs, int k, int R, dim3 grid, dim3 blk, hipStream_t s) {
+  switch (k) {
+    case 1: hipLaunchKernelGGL((k_root<1, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 2: hipLaunchKernelGGL((k_root<2, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 3: hipLaunchKernelGGL((k_root<3, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 4: hipLaunchKernelGGL((k_root<4, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 5: hipLaunchKernelGGL((k_root<5, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 6: hipLaunchKernelGGL((k_root<6, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 7: hipLaunchKernelGGL((k_root<7, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    case 8: hipLaunchKernelGGL((k_root<8, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+    default: hipLaunchKernelGGL((k_root<kMaxRootChildren, LSE64>), grid, blk, 0, s, b, jobs, R); break;
+  }
+}
+
+hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, bool lse64,
+                       hipStream_t s) {
   if (n_jobs <= 0) return hipSuccess;
   if (k < 0 || k > kMaxRootChildren) return hipErrorInvalidValue;
   dim3 grid((max_hw + 255) / 256, n_jobs);
   const dim3 blk(256);
-  switch (k) {
-    case 1: hipLaunchKernelGGL(k_root<1>, grid, blk, 0, s, b, jobs, R); break;
-    case 2: hipLaunchKernelGGL(k_root<2>, grid, blk, 0, s, b, jobs, R); break;
-    case 3: hipLaunchKernelGGL(k_root<3>, grid, blk, 0, s, b, jobs, R); break;
-    case 4: hipLaunchKernelGGL(k_root<4>, grid, blk, 0, s, b, jobs, R); break;
-    case 5: hipLaunchKernelGGL(k_root<5>, grid, blk, 0, s, b, jobs, R); break;
-    case 6: hipLaunchKernelGGL(k_root<6>, grid, blk, 0, s, b, jobs, R); break;
-    case 7: hipLaunchKernelGGL(k_root<7>, grid, blk, 0, s, b, jobs, R); break;
-    case 8: hipLaunchKernelGGL(k_root<8>, grid, blk, 0, s, b, jobs, R); break;
-    default: hipLaunchKernelGGL(k_root<kMaxRootChildren>, grid, blk, 0, s, b, jobs, R); break;
-  }
+  if (lse64)
+    launch_root_k<true>(b, jobs, k, R, grid, blk, s);
+  else
+    launch_root_k<false>(b, jobs, k, R, grid, blk, s);
   return hipGetLastError();
 }
 

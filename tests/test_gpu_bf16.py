@@ -263,7 +263,10 @@ def test_method_fallback_for_wide_filters(gpu):
 def test_width_limit_of_the_mfma_passes(gpu):
     # 512 columns (8 full waves per row-pass block) is the widest grid the
     # MFMA passes take; a batch with a wider grid runs every level in FP32,
-    # with the FP32 method's results bit for bit
+    # with the FP32 method's marginals bit for bit (the root posterior's
+    # log-sum-exp stays the BF16 methods' shifted FP64 one: the same values
+    # where the FP32 sum of expf does not underflow to LOG_ZERO, below the
+    # underflow bound where it does)
     ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
     joints = synth_joints(140)
     model = person_model(ep, joints)
@@ -279,7 +282,10 @@ def test_width_limit_of_the_mfma_passes(gpu):
     assert used == 0 and levels == 4
     outs_fp32, _ = run_method(model, [u520], [joints], "fp32")
     assert np.array_equal(outs_mix[0][0], outs_fp32[0][0])
-    assert np.array_equal(outs_mix[0][1], outs_fp32[0][1])
+    r_mix, r_32 = outs_mix[0][1], outs_fp32[0][1]
+    in_range = r_32 > -1e5
+    np.testing.assert_allclose(r_mix[in_range], r_32[in_range], rtol=1e-5, atol=1e-5)
+    assert (r_mix[~in_range] < -80.0).all()
 
 
 def test_other_rotation_counts_run_fp32(gpu):

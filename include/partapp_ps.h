@@ -32,7 +32,7 @@ typedef enum ps_status {
   PS_ERR_COVARIANCE = -3,    /* C diagonal not > 0 (multi_array_filter.hpp:219) */
   PS_ERR_KERNEL_SIZE = -4,   /* Gaussian kernel >= 1000 taps (multi_array_filter.hpp:243-244) */
   PS_ERR_UNSUPPORTED = -5,   /* e.g. R > 64, more than 16 root children      */
-  PS_ERR_WARP = -6,          /* a nearest-neighbour warp is not a uniform shift over the grid */
+  PS_ERR_WARP = -6,          /* a nearest-neighbour warp's offset changes more than once along an axis */
   PS_ERR_HIP = -7,           /* HIP runtime failure                          */
   PS_ERR_NO_DEVICE = -8,     /* no usable gfx950 device                      */
   PS_ERR_OUT_OF_MEMORY = -9,

@@ -32,10 +32,10 @@ size_t rowpass_lds_bytes(int R, int W, int kx);
 hipError_t rowpass_set_lds_limit(size_t bytes);
 hipError_t colpass_set_lds_limit(size_t bytes);
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s, int threads = 0);
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s, int threads, bool split);
 size_t colpass_lds_bytes(int ky, int rows);
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows = 0);
+                          int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows, bool split);
 hipError_t launch_offdiag(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts, int R,
                           long long max_cells, long long max_hw, hipStream_t s);
 hipError_t launch_root(const Bases &b, const RootJob *jobs, int n_jobs, int k, int R, int max_hw, bool lse64,
@@ -75,6 +75,7 @@ struct WidthClass {
   int row_threads = 0;      // tuned threads per block of the FP32 row pass (0: one per column)
   int col_rows = 0;         // tuned output rows per lane of the FP32 column pass (0: the default, 8)
   bool leaf = false;        // a leaf's message among its jobs (its source is a unary)
+  bool split = false;       // a job whose warp changes its offset along an axis (MsgJob::split)
   // the jobs of each instance: consecutive within the class (the level's
   // jobs keep instance order within a class); (instance, first job relative
   // to the level, count), by instance
@@ -85,6 +86,7 @@ struct WidthClass {
 void class_runs(WidthClass &c, const std::vector<MsgJob> &v) {
   for (int j = c.start; j < c.start + c.count; ++j) {
     if (v[j].src_kind == kSrcUnary) c.leaf = true;
+    if (v[j].split) c.split = true;
     if (c.run_inst.empty() || c.run_inst.back() != v[j].inst) {
       c.run_inst.push_back(v[j].inst);
       c.run_first.push_back(j);
@@ -106,6 +108,7 @@ struct Level {
   size_t mfma_col_lds[3] = {0, 0, 0};
   size_t mfma_col16_lds = 0;           // column pass on an fp16 row-pass output
   WidthClass od;                       // the off-diagonal jobs [n_diag, count), for their instance runs
+  bool split = false;                  // some job's warp changes its offset along an axis
 };
 
 // Host bf16 of a float, round to nearest even (finite inputs).
@@ -437,6 +440,22 @@ struct Builder {
     // the source slice of each output slice clamped into [0, R) (an address
     // inside the grid for a slice without a source, whose loads are discarded)
     for (int ro = 0; ro < R; ++ro) shifts.push_back(std::max(0, std::min(R - 1, ro - mp.delta)));
+    // a warp whose offset changes along an axis: breaks and steps, forward by
+    // output slice (x break, x step, y break, y step; none for a slice
+    // without a source), inverse by slice
+    J.split = mp.split ? 1 : 0;
+    if (mp.split) {
+      const std::vector<int> *fwd[4] = {&mp.fwd_xb, &mp.fwd_xd, &mp.fwd_yb, &mp.fwd_yd};
+      const int none[4] = {J.W, 0, J.H, 0};
+      for (int k = 0; k < 4; ++k)
+        for (int ro = 0; ro < R; ++ro) {
+          const int rs = ro - mp.delta;
+          shifts.push_back(rs >= 0 && rs < R ? (*fwd[k])[rs] : none[k]);
+        }
+      for (const std::vector<int> *v : {&mp.inv_xb, &mp.inv_xd, &mp.inv_yb, &mp.inv_yd})
+        shifts.insert(shifts.end(), v->begin(), v->end());
+      J.kk_x = J.kk_y = 0;  // the BF16 passes take uniform warps only: FP32
+    }
     return PS_OK;
   }
 };
@@ -743,6 +762,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       class_runs(L.od, v);
       L.mfma_ok = R == 24 || R == 48;
       for (MsgJob &j : v) {
+        if (j.split) L.split = true;
         L.max_h = std::max(L.max_h, j.H);
         L.max_w = std::max(L.max_w, j.W);
         L.lds = std::max(L.lds, rowpass_lds_bytes(R, j.W, j.kx));
@@ -1115,12 +1135,13 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
                                 L.mfma_row_lds[ti], c.leaf, s);
       else
         e = launch_rowpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w, L.lds, s,
-                           c.row_threads);
+                           c.row_threads, c.split);
       profile_end(pp, ev2, s);
     }
     if (e == hipSuccess && n_od > 0) {  // off-diagonal messages: the FP32 row pass, then the eigenbasis stages
       const long ev2 = profile_begin(pp, 2, s);
-      e = launch_rowpass(b, jobs + L.n_diag, n_od, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s);
+      e = launch_rowpass(b, jobs + L.n_diag, n_od, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w, L.lds, s, 0,
+                         L.split);
       profile_end(pp, ev2, s);
     }
     if (e != hipSuccess) return;
@@ -1136,7 +1157,7 @@ static hipError_t sweep(ps_plan *pl, const Bases &b, int terms, int col_terms, p
                                 c.epi_hi, c.max_w, terms != 3 ? L.mfma_col16_lds : L.mfma_col_lds[tc], c.leaf, s);
       else
         e = launch_colpass(b, jobs + c.start, c.count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w, L.col_lds, s,
-                           c.col_rows);
+                           c.col_rows, c.split);
       profile_end(pp, ev2, s);
     }
     // the column pass class also times the eigenbasis stages
@@ -1240,20 +1261,20 @@ static hipError_t sweep_subset(ps_plan *pl, const Bases &b, ps_map *map_dev, hip
       const WidthClass &c = L.classes[k];
       const long ev2 = profile_begin(pp, 2, s);
       e = launch_rowpass(b, jobs + ls.row[k].start, ls.row[k].count, pl->taps, pl->shifts, pl->R, c.max_h, c.max_w,
-                         L.lds, s, c.row_threads);
+                         L.lds, s, c.row_threads, c.split);
       profile_end(pp, ev2, s);
     }
     if (e == hipSuccess && ls.od_row.count > 0) {
       const long ev2 = profile_begin(pp, 2, s);
       e = launch_rowpass(b, jobs + ls.od_row.start, ls.od_row.count, pl->taps, pl->shifts, pl->R, L.max_h, L.max_w,
-                         L.lds, s);
+                         L.lds, s, 0, L.split);
       profile_end(pp, ev2, s);
     }
     for (size_t k = 0; k < L.col_classes.size() && e == hipSuccess; ++k) {
       const WidthClass &c = L.col_classes[k];
       const long ev2 = profile_begin(pp, 3, s);
       e = launch_colpass(b, jobs + ls.col[k].start, ls.col[k].count, pl->taps, pl->shifts, pl->R, c.epi_hi, c.max_w,
-                         L.col_lds, s, c.col_rows);
+                         L.col_lds, s, c.col_rows, c.split);
       profile_end(pp, ev2, s);
     }
     if (e == hipSuccess && ls.od_col.count > 0) {
@@ -1484,6 +1505,7 @@ int message_prepare(const ps_model *model, int height, int width, const double o
   ms.H = height;
   ms.W = width;
   ms.od = mp.od != 0;
+  ms.split = mp.split;
   ms.od_cells = mp.od ? (long long)R * mp.th * mp.tw : 0;
   ms.lds = rowpass_lds_bytes(R, width, mj.kx);
   ms.col_lds = mp.od ? 0 : colpass_lds_bytes(mp.ky, 16);
@@ -1547,12 +1569,12 @@ int message_enqueue(const MessageSetup &ms, char *mem, const float *child_dev, f
   if (e == hipSuccess) e = launch_reset(b, 1, 1, 1, nullptr, 0, s);
   if (e == hipSuccess)
     e = launch_max(b, reinterpret_cast<const MaxJob *>(mem + ms.o_max), 1, ms.R, ms.H * ms.W, s);
-  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, ms.R, ms.H, ms.W, ms.lds, s);
+  if (e == hipSuccess) e = launch_rowpass(b, jd, 1, td, sd, ms.R, ms.H, ms.W, ms.lds, s, 0, ms.split);
   if (e == hipSuccess) {
     if (ms.od)
       e = launch_offdiag(b, jd, 1, td, sd, ms.R, ms.od_cells, (long long)ms.H * ms.W, s);
     else
-      e = launch_colpass(b, jd, 1, td, sd, ms.R, ms.epi, ms.W, ms.col_lds, s);
+      e = launch_colpass(b, jd, 1, td, sd, ms.R, ms.epi, ms.W, ms.col_lds, s, 0, ms.split);
   }
   // the caller's tables live in pageable host memory and the block may be
   // reused right after: the call is synchronous

@@ -56,7 +56,7 @@ enum RotMode : int { kRotFilter = 0, kRotCopy = 1, kRotZero = 2 };
 struct MsgJob {
   int H, W;
   int inst;      // instance of the plan
-  int pad0_;
+  int split;     // 1: a warp's offset changes once along an axis (breaks and steps in the shift pool)
   int delta;     // rotation shift: output slice = source slice + delta
   int rot_mode;  // RotMode
   int rot_len;   // taps of the circular rotation filter
@@ -65,7 +65,9 @@ struct MsgJob {
   int taps_x;    //   with zeros to a multiple of 16
   int taps_y;
   int shifts;    // offset into the shift pool: fwd_ox[R] fwd_oy[R] inv_ox[R] inv_oy[R] (by source
-                 // slice), then fwd_ox[R] fwd_oy[R] by output slice, then those packed in 16 bits each
+                 // slice), then fwd_ox[R] fwd_oy[R] by output slice, then those packed in 16 bits each,
+                 // then the clamped source slice by output slice; split: from 8 R the forward x break,
+                 // x step, y break, y step by output slice, then the inverse ones by slice
   int src_kind;  // SrcKind
   int src_key;   // index of the source's max key
   GridRef src;   // [R][H][W]

@@ -268,11 +268,28 @@ __device__ __forceinline__ void xfilter_rows(const float *P, const float *taps_x
   }
 }
 
+// Forward offsets of output slice ro (so = J.shifts + 4 R: the x offsets at
+// so, the y offsets at so + R) at row ys / column x.  SPLIT: the warp's
+// offset changes once along an axis, by the step at so + 5 R (x) / so + 7 R
+// (y) from the break at so + 4 R / so + 6 R on.
+template <int R, bool SPLIT>
+__device__ __forceinline__ int fwd_oy(const int *__restrict__ shifts, int so, int ro, int ys) {
+  int oy = shifts[so + R + ro];
+  if (SPLIT && ys >= shifts[so + 6 * R + ro]) oy += shifts[so + 7 * R + ro];
+  return oy;
+}
+template <int R, bool SPLIT>
+__device__ __forceinline__ int fwd_ox(const int *__restrict__ shifts, int so, int ro, int x) {
+  int ox = shifts[so + ro];
+  if (SPLIT && x >= shifts[so + 4 * R + ro]) ox += shifts[so + 5 * R + ro];
+  return ox;
+}
+
 // One column x of the fixed-R row pass for slice group G: the R source
 // values are loaded together, exponentiated and rotation-filtered in
 // registers; the group's output slices G * RG .. (G + 1) * RG - 1 are written
 // to P.
-template <int R, int G>
+template <int R, int G, bool SPLIT>
 __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__restrict__ taps,
                                                const int *__restrict__ shifts, const float *src, float negM,
                                                float e_fill, float *P, int y, const RowGeom &g, const int x) {
@@ -303,9 +320,9 @@ __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__r
   const int x4 = x * 4, xmax4 = (W - 1) * 4;
 #pragma unroll
   for (int ro = 0; ro < R; ++ro) {
-    const int yy = min(max(ys + shifts[so + R + ro], 0), H - 1);
+    const int yy = min(max(ys + fwd_oy<R, SPLIT>(shifts, so, ro, ys), 0), H - 1);
     const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * W) * 4u;
-    const int xx4 = min(max(x4 + shifts[so + ro] * 4, 0), xmax4);
+    const int xx4 = min(max(x4 + fwd_ox<R, SPLIT>(shifts, so, ro, x) * 4, 0), xmax4);
     v[ro] = *reinterpret_cast<const float *>(srow + (unsigned)xx4);
   }
   // masks, then exp(b - M) (:362-365) as 2^((b - M) log2 e) on the hardware
@@ -322,14 +339,14 @@ __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__r
 #pragma unroll
   for (int ro = 0; ro < R; ++ro) {
     const int rs = ro - delta;
-    const bool row_ok = (unsigned)(ys + shifts[so + R + ro]) < (unsigned)H &&
+    const bool row_ok = (unsigned)(ys + fwd_oy<R, SPLIT>(shifts, so, ro, ys)) < (unsigned)H &&
                         !((unsigned)rs < (unsigned)R && ((rot_bits >> (rs & 63)) & 1ull));
     // a leaf without a detector has the zero log posterior (b = 0); the
     // exponential is computed for every lane (an empty asm keeps it out of a
     // branch), then selected
     float ev = __builtin_amdgcn_exp2f(((zero_src ? 0.0f : v[ro]) + negM) * kLog2eF);
     asm volatile("" : "+v"(ev));
-    const bool ok = row_ok && (unsigned)(x + shifts[so + ro] - lo) < span;
+    const bool ok = row_ok && (unsigned)(x + fwd_ox<R, SPLIT>(shifts, so, ro, x) - lo) < span;
     e[ro] = ok ? ev : e_fill;
   }
   float *out = P + kx + x;
@@ -347,7 +364,7 @@ __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__r
 // Row pass for a rotation count fixed at compile time (24, and the
 // reference's default 48): lane t owns the columns x = t (mod blockDim.x);
 // the slice groups go through P and the x filter one after another.
-template <int R>
+template <int R, bool SPLIT>
 __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__restrict__ jobs,
                                                    const float *__restrict__ taps, const int *__restrict__ shifts) {
   extern __shared__ float4 lds4[];
@@ -371,14 +388,20 @@ __global__ __launch_bounds__(512) void k_rowpass_r(Bases b, const MsgJob *__rest
   const float *src = cptr(b, J.src);
   float *T = mptr(b, J.tmp);
   const long long plane = (long long)H * J.tpitch;
-  for (int x = threadIdx.x; x < W; x += blockDim.x)
-    rowpass_column<R, 0>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+  // a warp whose offset changes along an axis takes the SPLIT variants
+  const bool split = SPLIT && J.split != 0;
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    if (split) rowpass_column<R, 0, true>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+    else rowpass_column<R, 0, false>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+  }
   __syncthreads();
   xfilter_rows(P, taps + J.taps_x, T, RG, J.tpitch, plane, y, g);
   if constexpr (row_groups(R) > 1) {
     __syncthreads();  // every item of the first group is filtered
-    for (int x = threadIdx.x; x < W; x += blockDim.x)
-      rowpass_column<R, 1>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+      if (split) rowpass_column<R, 1, true>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+      else rowpass_column<R, 1, false>(J, taps, shifts, src, negM, e_fill, P, y, g, x);
+    }
     __syncthreads();
     xfilter_rows(P, taps + J.taps_x, T + RG * plane, RG, J.tpitch, plane, y, g);
   }
@@ -418,8 +441,14 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
       const int rs = ro - J.delta;
       float e = e_fill;
       if (rs >= 0 && rs < R) {
-        const int yy = y + foy[rs];
-        const int xx = x + fox[rs];
+        int oy = foy[rs], ox = fox[rs];
+        if (J.split) {  // the breaks and steps by output slice
+          const int *sp = shifts + J.shifts + 8 * R;
+          if (x >= sp[ro]) ox += sp[R + ro];
+          if (y >= sp[2 * R + ro]) oy += sp[3 * R + ro];
+        }
+        const int yy = y + oy;
+        const int xx = x + ox;
         if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
           float v;
           if (J.src_kind == kSrcZero) v = 0.0f;
@@ -492,7 +521,9 @@ __device__ __forceinline__ void col_chunk(float (&acc)[ROWS], float (&buf)[ROWS]
 // epilogue operands first, then the next tile's rows (always loaded, clamped
 // into the grid, zeroed when they reach LDS), and the epilogue waits for its
 // operands only.
-template <int EPI, int ROWS>
+// SPLIT: the launch may hold a split warp (J.split), whose inverse offsets
+// change once along x (per lane) or y (a second segment of rows)
+template <int EPI, int ROWS, bool SPLIT>
 __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, const float *__restrict__ taps,
                                               const int *__restrict__ shifts, int R, int r, int xt, float *tile) {
   constexpr int kColRows = ROWS;
@@ -506,8 +537,14 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   const bool xin = x < W;
   const int xo = xin ? x : W - 1;
 
-  const int ox = shifts[J.shifts + 2 * R + r];
-  const int oy = shifts[J.shifts + 3 * R + r];
+  // inverse offsets of slice r (a split warp: the lane's column past the x
+  // break takes the x step, the rows from the y break on the y step)
+  int ox = shifts[J.shifts + 2 * R + r];
+  const int oy0 = shifts[J.shifts + 3 * R + r];
+  const bool split = SPLIT && J.split != 0;
+  if (split && x >= shifts[J.shifts + 12 * R + r]) ox += shifts[J.shifts + 13 * R + r];
+  const int ybrk = split ? shifts[J.shifts + 14 * R + r] : H;
+  const int ystep = split ? shifts[J.shifts + 15 * R + r] : 0;
   const bool col_ok = xin && x + ox >= 0 && x + ox < W;
   const int HW = H * W;
   int tp = J.tpitch;
@@ -530,116 +567,124 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   auto src_raw = [&](int row) -> float { return Tc[(long long)min(max(row, 0), H - 1) * tp]; };
   auto src_ok = [&](int row) -> bool { return lcol_ok && row >= 0 && row < H; };
 
-  // Tile t holds source rows s_t + [0, nrows), s_t = 64 t + oy - ky.  Tile t+1
-  // shares rows [64, nrows) of tile t (moved to [0, Lp) in LDS) and needs 64
-  // new rows, fetched into registers while tile t is filtered.
+  // The output rows run in segments of one inverse row offset oy: [0, ybrk)
+  // and, for a split warp, [ybrk, H).  Within a segment, tile t holds source
+  // rows s_t + [0, nrows), s_t = ylo + 64 t + oy - ky.  Tile t+1 shares rows
+  // [64, nrows) of tile t (moved to [0, Lp) in LDS) and needs 64 new rows,
+  // fetched into registers while tile t is filtered.
   constexpr int kNew = kColTileY / kColWaves;  // new rows per thread
-  for (int rr = wave; rr < nrows; rr += kColWaves) {
-    const int row = oy - ky + rr;
-    const float v = src_raw(row);
-    tile[rr * kColTileX + lane] = src_ok(row) ? v : 0.0f;
-  }
-  __syncthreads();
-
   Epi<EPI> acc;
   unsigned Wb = (unsigned)W * 4u;
-  for (int ty0 = 0; ty0 < H; ty0 += kColTileY) {
-    asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
-    const bool more = ty0 + kColTileY < H;
-    const int y0 = ty0 + wave * kColRows;
-    const unsigned eb0 = (unsigned)(y0 * W + xo) * 4u;
-    // the unary of the part written, loaded now so that its latency hides
-    // behind the filter; the stored belief of the five-grid downward epilogue
-    // is loaded after the filter (held through it, its 16 registers make the
-    // kernel spill at 128 VGPRs)
-    float pu[kColRows], pb[kColRows];
-    if (kNeedU || kNeedB) {
-      if (ty0 + kColTileY <= H) {
+  for (int seg = 0; seg < (SPLIT ? 2 : 1); ++seg) {
+    const int ylo = seg == 0 ? 0 : ybrk, yhi = seg == 0 ? ybrk : H;
+    if (ylo >= yhi) continue;
+    const int oy = seg == 0 ? oy0 : oy0 + ystep;
+    if (seg) __syncthreads();  // every wave is done with the first segment's tile
+    for (int rr = wave; rr < nrows; rr += kColWaves) {
+      const int row = ylo + oy - ky + rr;
+      const float v = src_raw(row);
+      tile[rr * kColTileX + lane] = src_ok(row) ? v : 0.0f;
+    }
+    __syncthreads();
+
+    for (int ty0 = ylo; ty0 < yhi; ty0 += kColTileY) {
+      asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
+      const bool more = ty0 + kColTileY < yhi;
+      const int y0 = ty0 + wave * kColRows;
+      const unsigned eb0 = (unsigned)(y0 * W + xo) * 4u;
+      // the unary of the part written, loaded now so that its latency hides
+      // behind the filter; the stored belief of the five-grid downward epilogue
+      // is loaded after the filter (held through it, its 16 registers make the
+      // kernel spill at 128 VGPRs)
+      float pu[kColRows], pb[kColRows];
+      if (kNeedU || kNeedB) {
+        if (ty0 + kColTileY <= H) {
 #pragma unroll
-        for (int i = 0; i < kColRows; ++i) {
-          const unsigned eb = eb0 + (unsigned)i * Wb;
-          if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
+          for (int i = 0; i < kColRows; ++i) {
+            const unsigned eb = eb0 + (unsigned)i * Wb;
+            if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < kColRows; ++i) {
+            const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
+            if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
+          }
+        }
+      }
+      // the next tile's new rows (in the last tile: clamped rows, unused)
+      const int s1 = ty0 + kColTileY + oy - ky + Lp;
+      float pre[kNew];
+#pragma unroll
+      for (int k = 0; k < kNew; ++k) pre[k] = src_raw(s1 + wave + kColWaves * k);
+
+      const float *tw = tile + wave * kColRows * kColTileX + lane;
+      float a[kColRows], buf[kColRows];
+#pragma unroll
+      for (int i = 0; i < kColRows; ++i) {
+        a[i] = 0.0f;
+        buf[i] = tw[i * kColTileX];
+      }
+      // taps in runs of the ring's period, then a tail (the tap count is
+      // padded to a multiple of 4 with zero taps)
+      int t0 = 0;
+      if constexpr (ROWS == 16) {
+        for (; t0 + 16 <= Lp; t0 += 16) {
+          col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
+          col_chunk<8, 8, ROWS>(a, buf, fy, tw, t0 + 8);
+        }
+        const int rem = Lp - t0;
+        if (rem >= 8) {
+          col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
+          if (rem == 12) col_chunk<8, 4, ROWS>(a, buf, fy, tw, t0 + 8);
+        } else if (rem == 4) {
+          col_chunk<0, 4, ROWS>(a, buf, fy, tw, t0);
         }
       } else {
+        for (; t0 + 8 <= Lp; t0 += 8) col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
+        if (Lp - t0 == 4) col_chunk<0, 4, ROWS>(a, buf, fy, tw, t0);
+      }
+
+      if (kNeedB) {
 #pragma unroll
         for (int i = 0; i < kColRows; ++i) {
           const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
-          if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
+          pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
         }
       }
-    }
-    // the next tile's new rows (in the last tile: clamped rows, unused)
-    const int s1 = ty0 + kColTileY + oy - ky + Lp;
-    float pre[kNew];
-#pragma unroll
-    for (int k = 0; k < kNew; ++k) pre[k] = src_raw(s1 + wave + kColWaves * k);
-
-    const float *tw = tile + wave * kColRows * kColTileX + lane;
-    float a[kColRows], buf[kColRows];
-#pragma unroll
-    for (int i = 0; i < kColRows; ++i) {
-      a[i] = 0.0f;
-      buf[i] = tw[i * kColTileX];
-    }
-    // taps in runs of the ring's period, then a tail (the tap count is
-    // padded to a multiple of 4 with zero taps)
-    int t0 = 0;
-    if constexpr (ROWS == 16) {
-      for (; t0 + 16 <= Lp; t0 += 16) {
-        col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
-        col_chunk<8, 8, ROWS>(a, buf, fy, tw, t0 + 8);
-      }
-      const int rem = Lp - t0;
-      if (rem >= 8) {
-        col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
-        if (rem == 12) col_chunk<8, 4, ROWS>(a, buf, fy, tw, t0 + 8);
-      } else if (rem == 4) {
-        col_chunk<0, 4, ROWS>(a, buf, fy, tw, t0);
-      }
-    } else {
-      for (; t0 + 8 <= Lp; t0 += 8) col_chunk<0, 8, ROWS>(a, buf, fy, tw, t0);
-      if (Lp - t0 == 4) col_chunk<0, 4, ROWS>(a, buf, fy, tw, t0);
-    }
-
-    if (kNeedB) {
 #pragma unroll
       for (int i = 0; i < kColRows; ++i) {
-        const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
-        pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
+        const int y = y0 + i;
+        // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
+        const float msg = (col_ok && (unsigned)(y + oy) < (unsigned)H) ? log0(a[i]) + M : kLogZero;
+        if (xin && y < yhi)
+          acc.store(dst, dst2, eb0 + (unsigned)i * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask,
+                    leaf_u);
       }
-    }
+      if (more) {
+        // move rows [64, nrows) to [0, Lp), then store the prefetched rows
+        for (int c0 = 0; c0 < Lp; c0 += kColMove * kColWaves) {
+          float mv[kColMove];
+          __syncthreads();  // every wave is done with this tile
 #pragma unroll
-    for (int i = 0; i < kColRows; ++i) {
-      const int y = y0 + i;
-      // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
-      const float msg = (col_ok && (unsigned)(y + oy) < (unsigned)H) ? log0(a[i]) + M : kLogZero;
-      if (xin && y < H)
-        acc.store(dst, dst2, eb0 + (unsigned)i * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask,
-                  leaf_u);
-    }
-    if (more) {
-      // move rows [64, nrows) to [0, Lp), then store the prefetched rows
-      for (int c0 = 0; c0 < Lp; c0 += kColMove * kColWaves) {
-        float mv[kColMove];
-        __syncthreads();  // every wave is done with this tile
+          for (int k = 0; k < kColMove; ++k) {
+            const int rr = c0 + wave + kColWaves * k;
+            if (rr < Lp) mv[k] = tile[(kColTileY + rr) * kColTileX + lane];
+          }
+          __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kColMove; ++k) {
-          const int rr = c0 + wave + kColWaves * k;
-          if (rr < Lp) mv[k] = tile[(kColTileY + rr) * kColTileX + lane];
+          for (int k = 0; k < kColMove; ++k) {
+            const int rr = c0 + wave + kColWaves * k;
+            if (rr < Lp) tile[rr * kColTileX + lane] = mv[k];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kNew; ++k) {
+          const int row = s1 + wave + kColWaves * k;
+          tile[(Lp + wave + kColWaves * k) * kColTileX + lane] = src_ok(row) ? pre[k] : 0.0f;
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kColMove; ++k) {
-          const int rr = c0 + wave + kColWaves * k;
-          if (rr < Lp) tile[rr * kColTileX + lane] = mv[k];
-        }
       }
-#pragma unroll
-      for (int k = 0; k < kNew; ++k) {
-        const int row = s1 + wave + kColWaves * k;
-        tile[(Lp + wave + kColWaves * k) * kColTileX + lane] = src_ok(row) ? pre[k] : 0.0f;
-      }
-      __syncthreads();
     }
   }
   acc.flush(b, J, (long long)r * HW, lane);
@@ -651,7 +696,7 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
 // epilogue fits without spilling vector registers once its stored belief is
 // loaded after the filter (2.16 -> 1.33 ms per launch on 32 300x300x24
 // grids; unbounded, at 156 VGPRs and 3 waves per SIMD, it takes 1.65 ms).
-template <int HI, int ROWS>
+template <int HI, int ROWS, bool SPLIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_colpass(
     Bases b, const MsgJob *__restrict__ jobs, const float *__restrict__ taps, const int *__restrict__ shifts, int R,
     int xtiles) {
@@ -666,15 +711,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (xt * kColTileX >= J.W) return;
   const int kind = epi_kind(J);
   if (HI == kEpiUpRoot || kind == kEpiUpRoot)
-    colpass_strip<kEpiUpRoot, ROWS>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiUpRoot, ROWS, SPLIT>(b, J, taps, shifts, R, r, xt, tile);
   else if (HI == kEpiUpNode || kind == kEpiUpNode)
-    colpass_strip<kEpiUpNode, ROWS>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiUpNode, ROWS, SPLIT>(b, J, taps, shifts, R, r, xt, tile);
   else if (HI == kEpiUpNodeU || kind == kEpiUpNodeU)
-    colpass_strip<kEpiUpNodeU, ROWS>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiUpNodeU, ROWS, SPLIT>(b, J, taps, shifts, R, r, xt, tile);
   else if (HI == kEpiDownLeaf || kind == kEpiDownLeaf)
-    colpass_strip<kEpiDownLeaf, ROWS>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiDownLeaf, ROWS, SPLIT>(b, J, taps, shifts, R, r, xt, tile);
   else
-    colpass_strip<kEpiDownMid, ROWS>(b, J, taps, shifts, R, r, xt, tile);
+    colpass_strip<kEpiDownMid, ROWS, SPLIT>(b, J, taps, shifts, R, r, xt, tile);
 }
 
 // ----------------------------------------------------------------- k_root
@@ -913,41 +958,49 @@ hipError_t raise_lds_limit(const void *f, size_t bytes) {
 }
 
 hipError_t rowpass_set_lds_limit(size_t bytes) {
-  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<24>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<48>), bytes);
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<24, false>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<48, false>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<24, true>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_r<48, true>), bytes);
   if (e != hipSuccess) return e;
   return raise_lds_limit(reinterpret_cast<const void *>(&k_rowpass_any), bytes);
 }
 
-template <int ROWS>
+template <int ROWS, bool SPLIT>
 static hipError_t colpass_rows_set_lds_limit(size_t bytes) {
-  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpRoot, ROWS>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNode, ROWS>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNodeU, ROWS>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownLeaf, ROWS>), bytes);
-  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownMid, ROWS>), bytes);
+  hipError_t e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpRoot, ROWS, SPLIT>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNode, ROWS, SPLIT>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiUpNodeU, ROWS, SPLIT>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownLeaf, ROWS, SPLIT>), bytes);
+  if (e == hipSuccess) e = raise_lds_limit(reinterpret_cast<const void *>(&k_colpass<kEpiDownMid, ROWS, SPLIT>), bytes);
   return e;
 }
 
 // bytes: the LDS of the 16-row tile (the larger one)
 hipError_t colpass_set_lds_limit(size_t bytes) {
-  hipError_t e = colpass_rows_set_lds_limit<8>(bytes);
-  if (e == hipSuccess) e = colpass_rows_set_lds_limit<16>(bytes);
+  hipError_t e = colpass_rows_set_lds_limit<8, false>(bytes);
+  if (e == hipSuccess) e = colpass_rows_set_lds_limit<16, false>(bytes);
+  if (e == hipSuccess) e = colpass_rows_set_lds_limit<8, true>(bytes);
   return e;
 }
 
 // threads: block size (a multiple of 64; 0 = one thread per column, at most
 // 512); every loop of the row pass strides by the block size
+// split: some job's warp changes its offset along an axis (MsgJob::split)
 hipError_t launch_rowpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s, int threads = 0) {
+                          int R, int max_h, int max_w, size_t lds_bytes, hipStream_t s, int threads, bool split) {
   if (n_jobs <= 0) return hipSuccess;
   dim3 grid(max_h, n_jobs);
   const int nt = threads > 0 ? std::min(threads, std::max(64, (max_w + 63) / 64 * 64))
                              : std::min(512, (max_w + 63) / 64 * 64);
-  if (R == 24)
-    hipLaunchKernelGGL(k_rowpass_r<24>, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
-  else if (R == 48)  // the reference's default grid: 2.4x the generic pass's sweep rate at 300x300
-    hipLaunchKernelGGL(k_rowpass_r<48>, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
+  if (R == 24 && !split)
+    hipLaunchKernelGGL((k_rowpass_r<24, false>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
+  else if (R == 24)
+    hipLaunchKernelGGL((k_rowpass_r<24, true>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
+  else if (R == 48 && !split)  // the reference's default grid
+    hipLaunchKernelGGL((k_rowpass_r<48, false>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
+  else if (R == 48)
+    hipLaunchKernelGGL((k_rowpass_r<48, true>), grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts);
   else
     hipLaunchKernelGGL(k_rowpass_any, grid, dim3(nt), lds_bytes, s, b, jobs, taps, shifts, R);
   return hipGetLastError();
@@ -958,16 +1011,16 @@ size_t colpass_lds_bytes(int ky, int rows) {
   return sizeof(float) * (size_t)col_tile_rows(ky, rows > 0 ? rows : 16) * kColTileX;
 }
 
-template <int ROWS>
+template <int ROWS, bool SPLIT>
 static void launch_colpass_rows(int epi_hi, dim3 grid, size_t lds_bytes, hipStream_t s, const Bases &b,
                                 const MsgJob *jobs, const float *taps, const int *shifts, int R, int xtiles) {
   const dim3 blk(kColWaves * 64);
   switch (epi_hi) {
-    case kEpiUpRoot: hipLaunchKernelGGL((k_colpass<kEpiUpRoot, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    case kEpiUpNode: hipLaunchKernelGGL((k_colpass<kEpiUpNode, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass<kEpiUpNodeU, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass<kEpiDownLeaf, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
-    default: hipLaunchKernelGGL((k_colpass<kEpiDownMid, ROWS>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpRoot: hipLaunchKernelGGL((k_colpass<kEpiUpRoot, ROWS, SPLIT>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpNode: hipLaunchKernelGGL((k_colpass<kEpiUpNode, ROWS, SPLIT>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiUpNodeU: hipLaunchKernelGGL((k_colpass<kEpiUpNodeU, ROWS, SPLIT>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    case kEpiDownLeaf: hipLaunchKernelGGL((k_colpass<kEpiDownLeaf, ROWS, SPLIT>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
+    default: hipLaunchKernelGGL((k_colpass<kEpiDownMid, ROWS, SPLIT>), grid, blk, lds_bytes, s, b, jobs, taps, shifts, R, xtiles); break;
   }
 }
 
@@ -975,18 +1028,22 @@ static void launch_colpass_rows(int epi_hi, dim3 grid, size_t lds_bytes, hipStre
 // output rows per lane (8 or 16; 0 = the default); lds_bytes_16: the LDS of
 // the 16-row tile for the jobs' widest y filter (the 8-row tile takes 32
 // rows of 256 bytes less)
+// (split: some job's warp changes its offset along an axis; those launches
+// take 8 rows per lane)
 hipError_t launch_colpass(const Bases &b, const MsgJob *jobs, int n_jobs, const float *taps, const int *shifts,
-                          int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows) {
+                          int R, int epi_hi, int max_w, size_t lds_bytes_16, hipStream_t s, int rows, bool split) {
   if (n_jobs <= 0) return hipSuccess;
-  if (rows == 0) rows = kColRowsDefault;
+  if (rows == 0 || split) rows = kColRowsDefault;
   if (epi_hi < kEpiUpRoot || epi_hi > kEpiDownMid || (rows != 8 && rows != 16)) return hipErrorInvalidValue;
   const int xtiles = (max_w + kColTileX - 1) / kColTileX;
   dim3 grid(R * xtiles, n_jobs);
-  if (rows == 16)
-    launch_colpass_rows<16>(epi_hi, grid, lds_bytes_16, s, b, jobs, taps, shifts, R, xtiles);
+  const size_t lds_8 = lds_bytes_16 - sizeof(float) * kColWaves * 8 * kColTileX;
+  if (split)
+    launch_colpass_rows<8, true>(epi_hi, grid, lds_8, s, b, jobs, taps, shifts, R, xtiles);
+  else if (rows == 16)
+    launch_colpass_rows<16, false>(epi_hi, grid, lds_bytes_16, s, b, jobs, taps, shifts, R, xtiles);
   else
-    launch_colpass_rows<8>(epi_hi, grid, lds_bytes_16 - sizeof(float) * kColWaves * 8 * kColTileX, s, b, jobs, taps,
-                           shifts, R, xtiles);
+    launch_colpass_rows<8, false>(epi_hi, grid, lds_8, s, b, jobs, taps, shifts, R, xtiles);
   return hipGetLastError();
 }
 

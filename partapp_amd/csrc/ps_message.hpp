@@ -17,6 +17,7 @@ namespace ps {
 struct MessageSetup {
   int R = 0, H = 0, W = 0;
   bool od = false;          // off-diagonal covariance: eigenbasis stages instead of the column pass
+  bool split = false;       // a warp whose offset changes along an axis (MsgJob::split)
   long long od_cells = 0;   // R * th * tw of the eigenbasis grid
   size_t lds = 0, col_lds = 0;
   // byte offsets in the block: job, max job, taps, shifts, max key, argmax key, grids

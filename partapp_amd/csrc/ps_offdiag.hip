@@ -197,6 +197,7 @@ __global__ __launch_bounds__(kOdThreads) void k_od_yfilter(Bases b, const MsgJob
 
 // Back to the grid: pixel (x, y) of slice r reads the smoothed slice at
 // (x + inv_ox[r], y + inv_oy[r]) (the inverse nearest warp, LOG_ZERO outside,
+// with a split warp's step past its break,
 // objectdetect_findrot.cpp:437-448), which is the bilinear sample of the
 // eigenbasis grid at T_back of that pixel, log, +M (:432-435); then the
 // caller's accumulation.
@@ -216,8 +217,12 @@ __global__ __launch_bounds__(kOdThreads) void k_od_back(Bases b, const MsgJob *_
   const int y = rem / W;
   const int x = rem - y * W;
   const int *sh = shifts + J.shifts;
-  const int xs = x + sh[2 * R + r];
-  const int ys = y + sh[3 * R + r];
+  int xs = x + sh[2 * R + r];
+  int ys = y + sh[3 * R + r];
+  if (J.split) {  // the inverse offsets change once along x / y: breaks and steps from 12 R on
+    if (x >= sh[12 * R + r]) xs += sh[13 * R + r];
+    if (y >= sh[14 * R + r]) ys += sh[15 * R + r];
+  }
   const float M = key2f(b.max_keys[J.src_key]);
   float *dst = mptr(b, J.dst);
   const float *dun = J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : nullptr;

@@ -3,9 +3,10 @@
 // The integer plan of a message reproduces the reference's double-precision
 // coordinate arithmetic exactly (objectdetect_findrot.cpp:316-359, 437-448;
 // homogeneous_coord.cpp:51-104; multi_array_transform.hpp:193-216), so the
-// nearest-neighbour warps become one integer shift per rotation slice; a
-// shift that is not uniform over the grid is reported as PS_ERR_WARP rather
-// than silently approximated.
+// nearest-neighbour warps become one integer shift per rotation slice and
+// axis, or two where the double sum's rounding changes it once along the
+// axis (a shift within an ulp of a half-integer); a shift that changes more
+// than once is reported as PS_ERR_WARP rather than silently approximated.
 #include "ps_plan.hpp"
 
 #include <cmath>
@@ -83,15 +84,21 @@ static void rotated_offset(const ps_exp_param &ep, int r, double scale, const do
 // output i is floor((i + shift) + 0.5) where shift = -d (forward warp, the
 // inverse of a translation by +d) or +d (inverse warp).  Returns false when
 // index - i is not the same for every i.
-static bool uniform_offset(double shift, int n, int &off) {
-  int o0 = 0;
-  for (int i = 0; i < n; ++i) {
-    double p = (double)i + shift;
-    int o = (int)std::floor(p + 0.5) - i;
-    if (i == 0) o0 = o;
-    else if (o != o0) return false;
+// floor(i + shift + 0.5) - i for i in [0, n) (multi_array_transform.hpp:
+// 197-205 on a translation): off below index brk, off + d from brk on
+// (brk = n, d = 0 when it is one value); false if it changes more than once
+static bool axis_offset(double shift, int n, int &off, int &brk, int &d) {
+  off = (int)std::floor(shift + 0.5);
+  brk = n;
+  d = 0;
+  for (int i = 1; i < n; ++i) {
+    const double p = (double)i + shift;
+    const int o = (int)std::floor(p + 0.5) - i;
+    if (o == (i < brk ? off : off + d)) continue;
+    if (brk != n) return false;
+    brk = i;
+    d = o - off;
   }
-  off = o0;
   return true;
 }
 
@@ -304,17 +311,25 @@ int build_message_plan(const ps_exp_param &ep, int H, int W, const double offset
     out.ky = ((int)fy.size() - 1) / 2;
   }
 
-  out.fwd_ox.assign(R, 0);
-  out.fwd_oy.assign(R, 0);
-  out.inv_ox.assign(R, 0);
-  out.inv_oy.assign(R, 0);
+  for (auto *v : {&out.fwd_ox, &out.fwd_oy, &out.inv_ox, &out.inv_oy, &out.fwd_xd, &out.fwd_yd, &out.inv_xd,
+                  &out.inv_yd})
+    v->assign(R, 0);
+  out.fwd_xb.assign(R, W);
+  out.inv_xb.assign(R, W);
+  out.fwd_yb.assign(R, H);
+  out.inv_yb.assign(R, H);
   for (int r = 0; r < R; ++r) {
     double dx, dy;
     rotated_offset(ep, r, scale, offset_c, dx, dy);
-    bool ok = uniform_offset(-dx, W, out.fwd_ox[r]) && uniform_offset(-dy, H, out.fwd_oy[r]);
+    bool ok = axis_offset(-dx, W, out.fwd_ox[r], out.fwd_xb[r], out.fwd_xd[r]) &&
+              axis_offset(-dy, H, out.fwd_oy[r], out.fwd_yb[r], out.fwd_yd[r]);
     rotated_offset(ep, r, scale, offset_p, dx, dy);
-    ok = ok && uniform_offset(dx, W, out.inv_ox[r]) && uniform_offset(dy, H, out.inv_oy[r]);
-    if (!ok) return set_error(PS_ERR_WARP, "nearest-neighbour warp is not a uniform shift at rotation " + std::to_string(r));
+    ok = ok && axis_offset(dx, W, out.inv_ox[r], out.inv_xb[r], out.inv_xd[r]) &&
+         axis_offset(dy, H, out.inv_oy[r], out.inv_yb[r], out.inv_yd[r]);
+    if (!ok)
+      return set_error(PS_ERR_WARP, "nearest-neighbour warp changes its offset more than once at rotation " +
+                                        std::to_string(r));
+    if (out.fwd_xb[r] < W || out.fwd_yb[r] < H || out.inv_xb[r] < W || out.inv_yb[r] < H) out.split = true;
   }
   return PS_OK;
 }

@@ -44,6 +44,14 @@ struct MessagePlan {
   int kx = 0, ky = 0;
   std::vector<int> fwd_ox, fwd_oy;   // [R] by source slice
   std::vector<int> inv_ox, inv_oy;   // [R] by slice
+  // A nearest-neighbour warp whose offset changes once along an axis (the
+  // double sum i + shift rounds differently in different binades of its
+  // magnitude when shift lies within an ulp of a half-integer): from index
+  // *_b on, the offset is the one above plus *_d.  b = the axis length, d = 0:
+  // uniform.  split: some slice's warp changes along some axis.
+  std::vector<int> fwd_xb, fwd_xd, fwd_yb, fwd_yd;  // [R] by source slice
+  std::vector<int> inv_xb, inv_xd, inv_yb, inv_yd;  // [R] by slice
+  bool split = false;
   // off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:334-369):
   // rotate into the eigenbasis of scale^2 C onto a th x tw grid, filter
   // there with fx, fy (kx, ky above are the eigenbasis half widths), sample back

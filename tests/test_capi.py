@@ -137,6 +137,21 @@ def test_plan_error_codes():
     assert st == 0 and p["rot_taps"] == []
 
 
+@pytest.mark.parametrize("oc,op", [((-3.5302013092419844, 0.0), (0.0, 0.0)), ((0.0, -3.5302013092419844), (0.0, 0.0)),
+                                   ((0.0, 0.0), (-3.5302013092419813, 0.0)), ((0.0, 0.0), (0.0, -3.5302013092419813))])
+def test_plan_takes_warps_whose_offset_changes_once(oc, op):
+    # offsets within an ulp of a half-integer at slice 0: the oracle's per-pixel
+    # warp changes its offset along one axis; the plan takes it (the offsets at
+    # index 0 are the oracle's; the kernels take the break, GPU-tested)
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    ref = oracle.message_plan(oracle.exp_param_from(ep), 24, 41, 57, oc, op, 0.0, 0.25, 1.0)
+    assert not ref["uniform"]
+    st, mine = _debug_plan(ep, 41, 57, oc, op, [30.0, 0, 0, 40.0], 0.0, 0.25, 1.0)
+    assert st == 0
+    for k in ("delta", "fwd_ox", "fwd_oy", "inv_ox", "inv_oy"):
+        assert mine[k] == ref[k], k
+
+
 def test_plan_create_without_gpu_reports_no_device():
     import torch
     if torch.cuda.is_available():

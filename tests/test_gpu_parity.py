@@ -105,6 +105,17 @@ MESSAGE_CASES = {
     "clipped_rot_kernel": dict(offset_c=(3.0, 4.0), offset_p=(-2.0, 1.0), C=(20.0, 0, 0, 20.0), rot_mean=0.1,
                                rot_sigma=3.0, scale=0.8),
 }
+# Offsets whose rotated value at slice 0 lies within an ulp of a half-integer:
+# the nearest-neighbour warp's offset changes once along the axis (the double
+# sum x + offset rounds differently in different binades), forward (offset_c)
+# or inverse (offset_p), along x or along y (at 41 x 57, found by the oracle's
+# per-pixel evaluation)
+SPLIT_C = -3.5302013092419844
+SPLIT_P = -3.5302013092419813
+for _name, _oc, _op in (("split_fwd_x", (SPLIT_C, 0.0), (0.0, 0.0)), ("split_fwd_y", (0.0, SPLIT_C), (0.0, 0.0)),
+                        ("split_inv_x", (0.0, 0.0), (SPLIT_P, 0.0)), ("split_inv_y", (0.0, 0.0), (0.0, SPLIT_P))):
+    MESSAGE_CASES[_name] = dict(offset_c=_oc, offset_p=_op, C=(30.0, 0, 0, 40.0), rot_mean=0.0, rot_sigma=0.25,
+                                scale=1.0)
 
 
 @pytest.mark.parametrize("case", sorted(MESSAGE_CASES))
@@ -113,6 +124,9 @@ def test_message_matches_oracle(gpu, case):
     kw = MESSAGE_CASES[case]
     R, H, W = 24, 41, 57
     ep = ExpParam(-180.0, 180.0, R, 1.0, 1.0, 1, 0.0)
+    plan = oracle.message_plan(oracle.exp_param_from(ep), R, H, W, kw["offset_c"], kw["offset_p"], kw["rot_mean"],
+                               kw["rot_sigma"], kw["scale"])
+    assert plan["uniform"] == (not case.startswith("split_"))
     model = person_model(ep)
     rng = np.random.default_rng(sum(map(ord, case)))
     child = rng.normal(-3, 1.0, (R, H, W)).astype(np.float32)
@@ -148,6 +162,34 @@ def test_sweep_small_matches_oracle(gpu, seed):
     say("sweep small seed", seed)
     out = run_plan(model, [u])[0]
     check_parity("small seed %d" % seed, model, u, joints, out)
+
+
+@pytest.mark.parametrize("method", ["fp32", "bf16mix"])
+def test_sweep_with_split_warps(gpu, method):
+    # two joints whose warps change their offset once (forward along x, inverse
+    # along y): the FP32 passes take their breaks; the BF16 methods run those
+    # levels on the FP32 passes
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = synth_joints(5)
+    j0, j5 = joints[0], joints[5]
+    joints[0] = Joint(j0.child_idx, j0.parent_idx, (SPLIT_C, 0.0), j0.offset_p, j0.C, j0.rot_mean, j0.rot_sigma)
+    joints[5] = Joint(j5.child_idx, j5.parent_idx, j5.offset_c, (0.0, SPLIT_P), j5.C, j5.rot_mean, j5.rot_sigma)
+    model = person_model(ep, joints)
+    u = synth_unary(5, 10, 1, 24, 96, 112)
+    say("sweep with split warps", method)
+    plan = Plan(model, [(96, 112)], [joints])
+    levels, used = plan.set_method(method)
+    plan.close()
+    if method == "fp32":
+        out = run_plan(model, [u])[0]
+        check_parity("split warps", model, u, joints, out)
+    else:
+        assert used < levels  # the levels of the two split messages run in FP32
+        from test_gpu_bf16 import METHOD_L1, errors_vs_f64, run_method
+        outs, _ = run_method(model, [u], [joints], method)
+        worst, map_bad = errors_vs_f64(model, u, joints, outs[0])
+        say("  bf16mix: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (worst, map_bad))
+        assert worst <= METHOD_L1[method] and not map_bad
 
 
 # trees other than the person tree, by parent of each part (-1 = root): the

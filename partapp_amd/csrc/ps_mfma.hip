@@ -186,11 +186,11 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
   int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
   int delta = J.delta;
-  // forward offsets by output slice (the source slice is ro - delta), packed
-  // x | y << 16; read through the scalar cache per row (the offset is opaque
-  // to the loop optimiser so that they do not occupy scalar registers
-  // throughout)
-  int so = J.shifts + 6 * R;
+  // forward offsets by output slice (the source slice is ro - delta): x at
+  // so, y at so + R, the source slice clamped into [0, R) at so + 3 R; read
+  // through the scalar cache per row (the offset is opaque to the loop
+  // optimiser so that they do not occupy scalar registers throughout)
+  int so = J.shifts + 4 * R;
   // The lane's column is re-read through an empty asm every row, so the loop
   // optimiser cannot hoist the per-slice addresses derived from it (2 R of
   // them) out of the row loop and hold them in registers for its lifetime.
@@ -201,26 +201,26 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   };
 
   // raw source values of row y (addresses clamped into the grid: every load
-  // is unconditional) and their masking when consumed; both read the 2 R
-  // offsets in one batch of scalar loads and are branch-free per slice
+  // is unconditional; the row address on the scalar unit, the column a byte
+  // offset from it) and their masking when consumed; both read the offsets
+  // through the scalar cache and are branch-free per slice
   auto offsets = [&](int (&ox)[R], int (&oy)[R]) {
+    asm volatile("" : "+s"(so));
 #pragma unroll
     for (int ro = 0; ro < R; ++ro) {
-      const int pk = shifts[so + ro];
-      ox[ro] = (int)(short)(pk & 0xffff);
-      oy[ro] = pk >> 16;
+      ox[ro] = shifts[so + ro];
+      oy[ro] = shifts[so + R + ro];
     }
   };
   auto gather = [&](int y, float (&v)[R]) {
-    int ox[R], oy[R];
-    offsets(ox, oy);
-    const int xl = min(lane_column(), W - 1);
+    asm volatile("" : "+s"(so), "+s"(HW));
+    const int x4 = lane_column() * 4, xmax4 = (W - 1) * 4;
 #pragma unroll
     for (int ro = 0; ro < R; ++ro) {
-      const int rs = min(max(ro - delta, 0), R - 1);
-      const int yy = min(max(y + oy[ro], 0), H - 1);
-      const int xx = min(max(xl + ox[ro], 0), W - 1);
-      v[ro] = src[rs * HW + yy * W + xx];
+      const int yy = min(max(y + shifts[so + R + ro], 0), H - 1);
+      const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * W) * 4u;
+      const int xx4 = min(max(x4 + shifts[so + ro] * 4, 0), xmax4);
+      v[ro] = *reinterpret_cast<const float *>(srow + (unsigned)xx4);
     }
   };
   // exp(b - M) = 2^(b log2 e - sub) of row y into the LDS rows, sub = M log2 e
@@ -260,11 +260,22 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
       }
     }
     if (x < W) {
+      // LDS addresses of slice ro's entries, advanced by one row pitch per
+      // slice (an empty asm keeps the R addresses from being formed up front
+      // and held in scalar registers)
+      typedef __attribute__((address_space(3))) unsigned short lds_u16;
+      lds_u16 *qh = (lds_u16 *)(Ph + kx + x), *ql = (lds_u16 *)(Pl + kx + x);
 #pragma unroll
       for (int ro = 0; ro < R; ++ro) {
         const __bf16 hi = (__bf16)e[ro];
-        Ph[ro * pitch + kx + x] = __builtin_bit_cast(unsigned short, hi);
-        if (TERMS >= 2) Pl[ro * pitch + kx + x] = bf16_bits(e[ro] - (float)hi);
+        *qh = __builtin_bit_cast(unsigned short, hi);
+        qh += pitch;
+        asm volatile("" : "+v"(qh));
+        if (TERMS >= 2) {
+          *ql = bf16_bits(e[ro] - (float)hi);
+          ql += pitch;
+          asm volatile("" : "+v"(ql));
+        }
       }
     }
   };
@@ -367,7 +378,8 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
 #pragma unroll
     for (int k = 0; k < NRES; ++k) {
       const int i = k < 16 ? k : k - 16;
-      char *p = Trow + ob + (unsigned)((k < 16 ? 0 : 32) + 8 * (i >> 2) + (i & 3)) * slice_b;
+      // a 32-bit offset from the row's (scalar) base
+      char *p = Trow + (ob + (unsigned)((k < 16 ? 0 : 32) + 8 * (i >> 2) + (i & 3)) * slice_b);
       if (T16)
         *reinterpret_cast<_Float16 *>(p) = (_Float16)(res[k] * t_scale);
       else

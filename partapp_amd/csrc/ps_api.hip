@@ -324,7 +324,7 @@ struct Builder {
   // ks reads the accumulator of slice tile ks / 2, so its B position holds
   // slice 32 (ks / 2) + 16 (ks % 2) + 8 (t / 4) + 4 h + t % 4; the table is
   // [mt][ks] per block.
-  int add_rot_frags(const MessagePlan &mp, int R) {
+  int add_rot_frags(const MessagePlan &mp, int R, bool f16 = false) {
     const int off = (int)(frags.size() / 8);
     std::vector<float> F((size_t)R * R, 0.0f);
     if (mp.rot_mode == kRotFilter) {
@@ -344,9 +344,15 @@ struct Builder {
             const int ro = 32 * mt + l % 32, h = l / 32;
             const int sl = 32 * (ks / 2) + 16 * (ks % 2) + 8 * (t / 4) + 4 * h + t % 4;
             const float v = (ro < R && sl < R) ? F[(size_t)ro * R + sl] : 0.0f;
-            const uint16_t hi = bf16_rne(v);
-            frags.push_back(hi);
-            lo.push_back(bf16_rne(v - bf16_float(hi)));
+            if (f16) {  // fp16 hi + lo
+              const _Float16 hi = (_Float16)v;
+              frags.push_back(__builtin_bit_cast(uint16_t, hi));
+              lo.push_back(__builtin_bit_cast(uint16_t, (_Float16)(v - (float)hi)));
+            } else {
+              const uint16_t hi = bf16_rne(v);
+              frags.push_back(hi);
+              lo.push_back(bf16_rne(v - bf16_float(hi)));
+            }
           }
     frags.insert(frags.end(), lo.begin(), lo.end());
     return off;
@@ -397,6 +403,7 @@ struct Builder {
         J.frag_y = add_frags(mp.fy, J.kk_y);
         J.frag_y16 = add_taps16(mp.fy, J.kk_y);
         J.frag_r = (R == 24 || R == 48) ? add_rot_frags(mp, R) : 0;
+        J.frag_r16 = (R == 24 || R == 48) ? add_rot_frags(mp, R, true) : 0;
         // the row-pass output is at most (sum of x taps) (largest row sum of
         // the rotation filter) times max exp(b - M) = 1; its fp16 copy holds
         // it scaled by 2^t16_shift, that bound (with margin) near 2^14
@@ -408,6 +415,9 @@ struct Builder {
           sr = 1.0;
         const double bound = 1.01 * sx * sr;
         J.t16_shift = bound > 0.0 ? std::max(-100, std::min(100, (int)std::floor(14.0 - std::log2(bound)))) : 0;
+        // the x filter's output, the rotation GEMM's fp16 operand there, is at
+        // most (sum of x taps) times max exp(b - M) = 1
+        J.x16_shift = sx > 0.0 ? std::max(-100, std::min(100, (int)std::floor(14.0 - std::log2(1.01 * sx)))) : 0;
       } else {
         J.kk_x = J.kk_y = 0;  // too wide for the BF16 passes: FP32 only
       }

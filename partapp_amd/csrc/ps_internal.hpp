@@ -92,6 +92,8 @@ struct MsgJob {
   int frag_r;  // rotation filter as the A operand of the row pass's second GEMM
   int t16_shift;  // BF16 passes with an fp16 row-pass output: stored as value * 2^t16_shift
   int frag_y16;   // the y taps as fp16 hi / lo tap copies (column pass on an fp16 row-pass output)
+  int frag_r16;   // the rotation filter as fp16 hi / lo A fragments (the row pass with an fp16 output)
+  int x16_shift;  // that row pass exponentiates to value * 2^x16_shift (its x filter's largest output near 2^14)
   // Off-diagonal covariance (gaussFilter2dOffset, multi_array_filter.hpp:
   // 334-369): the row pass writes the rotation-filtered grid only (kx = 0, a
   // unit x tap); each slice is then rotated onto a th x tw grid in the

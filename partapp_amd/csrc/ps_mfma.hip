@@ -57,6 +57,14 @@ __device__ __forceinline__ f32x16_t mfma(bf16x8_t a, bf16x8_t b, f32x16_t c) {
 __device__ __forceinline__ f32x16_t mfma16(f16x8_t a, f16x8_t b, f32x16_t c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ f16x8_t ld_frag16(const uint4 *p) { return *reinterpret_cast<const f16x8_t *>(p); }
+// eight floats rounded to fp16
+__device__ __forceinline__ f16x8_t to_f16x8(const float (&v)[8]) {
+  f16x8_t r;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) r[t] = (_Float16)v[t];
+  return r;
+}
 // eight floats as bf16 hi parts and lo parts (v - float(hi) is exact)
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8_t &hi, bf16x8_t &lo) {
 #pragma unroll
@@ -165,7 +173,8 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
 
   const uint4 *gf = frags + J.frag_x;
   for (int i = threadIdx.x; i < nf * KK * 64; i += blockDim.x) fr[i] = gf[i];
-  const uint4 *gr = frags + J.frag_r;
+  // the rotation taps: bf16 hi / lo, or fp16 hi / lo for an fp16 output
+  const uint4 *gr = frags + (T16 ? J.frag_r16 : J.frag_r);
   for (int i = threadIdx.x; i < nr * MT * KS * 64; i += blockDim.x) frr[i] = gr[i];
   // the halo columns stay zero for every row of the block
   const int per_row = pitch - W;
@@ -225,6 +234,7 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   };
   // exp(b - M) = 2^(b log2 e - sub) of row y into the LDS rows, sub = M log2 e
   // (M the source grid's maximum) or, for an fp16 output, the row's exponent
+  // less x16_shift (the values then carry that power of two)
   auto consume = [&](int y, const float (&v)[R], float sub) {
     int ox[R], oy[R];
     offsets(ox, oy);
@@ -233,12 +243,14 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
     if (zero_src) {
       // a leaf without a detector keeps the zero log posterior it was
       // allocated with (no unary added, objectdetect_findrot.cpp:652-654):
-      // exp(0 - M) = 1 (its max M is 0, and its row exponent) wherever the
-      // warp has a source pixel, exp(LOG_ZERO) = 0 elsewhere
+      // exp(0 - M) = 1 (its max M is 0, and its row exponent 0), i.e.
+      // 2^-sub, wherever the warp has a source pixel, exp(LOG_ZERO) = 0
+      // elsewhere
+      const float one = __builtin_amdgcn_exp2f(-sub);
 #pragma unroll
       for (int ro = 0; ro < R; ++ro) {
         const bool ok = (unsigned)(y + oy[ro]) < (unsigned)H && (unsigned)(x + ox[ro]) < (unsigned)W;
-        e[ro] = ok ? 1.0f : 0.0f;
+        e[ro] = ok ? one : 0.0f;
       }
     } else if (use_mask) {
       // no source pixel, or a masked unary state: LOG_ZERO (objectdetect_findrot.cpp:342-365)
@@ -290,9 +302,10 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   // byte offsets of this lane's outputs: slice 8 (i / 4) + 4 h + i % 4, column
   // 32 t + lane % 32, from the row's start
   // T16: the output is stored as fp16 of value * 2^t16_shift (the shift
-  // keeps the largest possible value near 2^14)
+  // keeps the largest possible value near 2^14); the rows were exponentiated
+  // with 2^x16_shift, which the store exchanges for it
   constexpr unsigned eb = T16 ? 2u : 4u;  // bytes per element
-  const float t_scale = T16 ? ldexpf(1.0f, J.t16_shift) : 1.0f;
+  const float t_scale = T16 ? ldexpf(1.0f, J.t16_shift - J.x16_shift) : 1.0f;
   const int tp = J.tpitch;
   unsigned slice_b = (unsigned)H * (unsigned)tp * eb;
   const unsigned ob0 = (unsigned)(4 * h) * slice_b + (unsigned)(32 * t0 + (lane & 31)) * eb;
@@ -330,14 +343,23 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
         float w[8];
 #pragma unroll
         for (int t8 = 0; t8 < 8; ++t8) w[t8] = c[ks >> 1][8 * (ks & 1) + t8];
-        bf16x8_t bh, bl;
-        split8(w, bh, bl);
+        if (T16) {  // data as fp16, taps fp16 hi + lo (two products)
+          const f16x8_t xb = to_f16x8(w);
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-          const bf16x8_t f = ld_frag(frr + 64 * (mt * KS + ks) + lane);
-          d[mt] = mfma(f, bh, d[mt]);
-          d[mt] = mfma(f, bl, d[mt]);
-          d[mt] = mfma(ld_frag(frr + 64 * (MT * KS) + 64 * (mt * KS + ks) + lane), bh, d[mt]);
+          for (int mt = 0; mt < 2; ++mt) {
+            d[mt] = mfma16(ld_frag16(frr + 64 * (mt * KS + ks) + lane), xb, d[mt]);
+            d[mt] = mfma16(ld_frag16(frr + 64 * (MT * KS) + 64 * (mt * KS + ks) + lane), xb, d[mt]);
+          }
+        } else {
+          bf16x8_t bh, bl;
+          split8(w, bh, bl);
+#pragma unroll
+          for (int mt = 0; mt < 2; ++mt) {
+            const bf16x8_t f = ld_frag(frr + 64 * (mt * KS + ks) + lane);
+            d[mt] = mfma(f, bh, d[mt]);
+            d[mt] = mfma(f, bl, d[mt]);
+            d[mt] = mfma(ld_frag(frr + 64 * (MT * KS) + 64 * (mt * KS + ks) + lane), bh, d[mt]);
+          }
         }
       }
 #pragma unroll
@@ -356,20 +378,28 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
       if (TERMS >= 2) c = mfma(ld_frag(al + 16 * kk), b_hi, c);
       if (TERMS == 3) c = mfma(a_hi, ld_frag(fr + (KK + kk) * 64 + lane), c);
     }
-    // rotation: k-step s takes accumulators 8 s .. 8 s + 7; data and taps
-    // split hi + lo (three products: hi*hi, hi*lo, lo*hi)
+    // rotation: k-step s takes accumulators 8 s .. 8 s + 7.  An fp16 output:
+    // data rounded to fp16 (the x filter's output at most ~2^14 there), taps
+    // split fp16 hi + lo (two products); else data and taps split bf16 hi +
+    // lo (three products: hi*hi, hi*lo, lo*hi)
     f32x16_t d = {};
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
       float w[8];
 #pragma unroll
       for (int t8 = 0; t8 < 8; ++t8) w[t8] = c[8 * st + t8];
-      bf16x8_t bh, bl;
-      split8(w, bh, bl);
-      const bf16x8_t f = ld_frag(frr + 64 * st + lane);
-      d = mfma(f, bh, d);
-      d = mfma(f, bl, d);
-      d = mfma(ld_frag(frr + 128 + 64 * st + lane), bh, d);
+      if (T16) {
+        const f16x8_t xb = to_f16x8(w);
+        d = mfma16(ld_frag16(frr + 64 * st + lane), xb, d);
+        d = mfma16(ld_frag16(frr + 128 + 64 * st + lane), xb, d);
+      } else {
+        bf16x8_t bh, bl;
+        split8(w, bh, bl);
+        const bf16x8_t f = ld_frag(frr + 64 * st + lane);
+        d = mfma(f, bh, d);
+        d = mfma(f, bl, d);
+        d = mfma(ld_frag(frr + 128 + 64 * st + lane), bh, d);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 12; ++i) res[i] = d[i];
@@ -432,7 +462,7 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
     if (T16) {
       const int ey = row_exponent();
       if (threadIdx.x == 0) rexp[y] = ey;
-      consume(y, v, (float)ey);
+      consume(y, v, (float)(ey - J.x16_shift));
     } else {
       consume(y, v, M * kLog2e);
     }

@@ -145,10 +145,10 @@ int check_device(int &device) {
   return PS_OK;
 }
 
-GridRef gref(int buf, long long off) {
+GridRef gref(int buf, long long off, int pitch) {
   GridRef g;
   g.buf = buf;
-  g.pad_ = 0;
+  g.pitch = pitch;
   g.off = off;
   return g;
 }
@@ -516,6 +516,13 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     }
   }
 
+  // inner parts (a parent and a child): each keeps its upward belief in a
+  // grid of its own until its downward message
+  std::vector<int> inner_idx(P, -1);
+  int n_inner = 0;
+  for (int p = 0; p < P; ++p)
+    if (p != t.root && t.child_joint[p] >= 0) inner_idx[p] = n_inner++;
+
   Builder B(m);
   const int items = n_instances * S;
   std::vector<std::vector<MsgJob>> up_jobs(t.max_height), dn_jobs(t.max_depth);
@@ -536,12 +543,29 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
     for (int s = 0; s < S; ++s) {
       const int it = i * S + s;
       const double scale = scale_from_index(m.ep, s);
-      auto unary = [&](int p) { return gref(kBufUnary, in.unary_offset + ((long long)p * S + s) * n); };
-      auto marg = [&](int p) { return gref(kBufMarginal, in.marginal_offset + ((long long)p * S + s) * n); };
-      const long long n64 = (n + 63) & ~63ll;  // grids start on 256-byte boundaries
-      const long long nt64 = ((long long)R * H * t_pitch(W) + 63) & ~63ll;  // row-pass outputs, padded rows
-      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * nt64); };
-      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + K * nt64 + (long long)k * n64); };
+      // the caller's grids are dense; the plan's own start every row on a
+      // 128-byte boundary (and every grid on a 256-byte one)
+      const int gp = grid_pitch(W);
+      auto unary = [&](int p) { return gref(kBufUnary, in.unary_offset + ((long long)p * S + s) * n, W); };
+      auto marg = [&](int p) { return gref(kBufMarginal, in.marginal_offset + ((long long)p * S + s) * n, W); };
+      const long long np64 = ((long long)R * H * gp + 63) & ~63ll;
+      const long long nt64 = ((long long)R * H * t_pitch(W) + 63) & ~63ll;  // row-pass outputs
+      auto tbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + (long long)k * nt64, t_pitch(W)); };
+      // message slot of chain k: the upward message into the root, then the
+      // downward sources of the chain past its first message
+      auto mbuf = [&](int k) { return gref(kBufScratch, (long long)scratch_off + K * nt64 + (long long)k * np64, gp); };
+      // upward belief of an inner part (a part with a parent and a child),
+      // kept from its upward message to its downward one
+      auto bel = [&](int p) {
+        return gref(kBufScratch, (long long)scratch_off + K * nt64 + (K + inner_idx[p]) * np64, gp);
+      };
+      // log_from_root of chain k (written by the root combine, read by the
+      // first downward message of the chain): dense like the caller's grids,
+      // so the root writes all its grids along its pixels
+      const long long n64 = (n + 63) & ~63ll;
+      auto lfr = [&](int k) {
+        return gref(kBufScratch, (long long)scratch_off + K * nt64 + (K + n_inner) * np64 + (long long)k * n64, W);
+      };
       auto up_key = [&](int p) { return it * 2 * P + p; };
       auto dn_key = [&](int p) { return it * 2 * P + P + p; };
       auto slot = [&](int p) { return it * P + p; };
@@ -592,10 +616,10 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       std::vector<long long> od_off(K, 0);
       long long od_total = 0;
       for (int k = 0; k < K; ++k) {
-        od_off[k] = (long long)scratch_off + K * nt64 + K * n64 + od_total;
+        od_off[k] = (long long)scratch_off + K * nt64 + (K + n_inner) * np64 + K * n64 + od_total;
         od_total += od_floats[k];
       }
-      auto odbuf = [&](int k) { return gref(kBufScratch, od_off[k]); };
+      auto odbuf = [&](int k) { return gref(kBufScratch, od_off[k], 0); };
 
       // upward levels, each message as late as its parent allows: the edge
       // into a node at depth d runs at level max_height - d, so every message
@@ -621,11 +645,12 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
             mj.src_mask = part_mask(m, c, W);
           } else {
             mj.src_kind = kSrcGrid;
-            mj.src = marg(c);
+            mj.src = bel(c);
           }
           mj.src_key = up_key(c);
           mj.tmp = tbuf(k);
           mj.dst2 = kNone;
+          mj.dst_bel = kNone;
           mj.dst_unary = kNone;
           keep_plane(mj, p);
           if (p == t.root) {
@@ -633,7 +658,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
             mj.dst = mbuf(k);
           } else {
             mj.dst_kind = kDstUpNode;
-            mj.dst = marg(p);
+            mj.dst = bel(p);
             mj.dst_unary = unary(p);
             mj.dst_mask = part_mask(m, p, W);
             mj.dst_add_unary = m.is_detect[p];
@@ -653,13 +678,14 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         rj.add_unary = m.is_detect[t.root];
         for (int k = 0; k < K; ++k) {
           rj.msg[k] = mbuf(k);
+          rj.fr[k] = lfr(k);
           rj.fr_key[k] = dn_key(chains[k][0]);
         }
         rj.unary = unary(t.root);
         rj.mask = part_mask(m, t.root, W);
         rj.marginal = marg(t.root);
         rj.map_slot = slot(t.root);
-        rj.root_post = gref(kBufRoot, in.root_offset + (long long)s * H * W);
+        rj.root_post = gref(kBufRoot, in.root_offset + (long long)s * H * W, W);
         if (m.is_upright[t.root]) {
           int k1 = index_from_rot(m.ep, -1e-6), k2 = index_from_rot(m.ep, 1e-6);
           if (k1 < 0 || k2 < 0 || k1 >= R || k2 >= R)
@@ -686,7 +712,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           B.add_message(mp, mj);
           mj.od_buf = mp.od ? odbuf(k) : kNone;
           mj.src_kind = kSrcGrid;
-          mj.src = mbuf(k);
+          mj.src = d == 1 ? lfr(k) : mbuf(k);
           mj.src_key = dn_key(c);
           mj.tmp = tbuf(k);
           mj.dst_kind = kDstDown;
@@ -696,7 +722,9 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
           mj.dst_mask = part_mask(m, c, W);
           const bool has_child = t.child_joint[c] >= 0;
           mj.dst_base = has_child ? 0 : (m.is_detect[c] ? 1 : 2);
+          mj.dst_bel = kNone;
           if (has_child) {
+            mj.dst_bel = bel(c);
             mj.dst2 = mbuf(k);
             mj.dst2_key = dn_key(m.joints[t.child_joint[c]].child_idx);
           } else {
@@ -716,7 +744,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         mj.out = it * P + p;
         B.maps.push_back(mj);
       }
-      scratch_off += (size_t)(K * nt64 + K * n64 + od_total);
+      scratch_off += (size_t)(K * nt64 + (K + n_inner) * np64 + K * n64 + od_total);
     }
   }
 
@@ -806,6 +834,25 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
   };
   st = add_levels(up_jobs, pl->up);
   if (st == PS_OK) st = add_levels(dn_jobs, pl->down);
+  // the kernels share one byte offset between grids of equal row pitch: the
+  // epilogues between dst and the unary a marginal adds, and between the next
+  // source and the stored belief; the root between its message slots, and
+  // between the unary and the marginal (dense, like the root posterior)
+  if (st == PS_OK) {
+    for (const MsgJob &j : B.jobs) {
+      const int kind = epi_kind(j);
+      bool ok = j.src.pitch >= j.W && j.dst.pitch >= j.W;
+      if (kind >= kEpiDownLeaf) ok = ok && j.dst_unary.pitch == j.dst.pitch;
+      if (kind == kEpiDownMid) ok = ok && j.dst_bel.pitch == j.dst2.pitch;
+      if (!ok) st = set_error(PS_ERR_INVALID_ARG, "internal: grid pitches of a message do not pair up");
+    }
+    for (const RootJob &j : B.roots) {
+      bool ok = j.unary.pitch == j.W && j.marginal.pitch == j.W;
+      for (int k = 0; k < j.K; ++k)
+        ok = ok && j.msg[k].pitch == j.msg[0].pitch && j.msg[k].pitch >= j.W && j.fr[k].pitch == j.W;
+      if (!ok) st = set_error(PS_ERR_INVALID_ARG, "internal: grid pitches of a root do not pair up");
+    }
+  }
   if (st != PS_OK) {
     delete pl;
     return st;
@@ -1522,15 +1569,16 @@ int message_prepare(const ps_model *model, int height, int width, const double o
   if (ms.lds > kLdsLimit || ms.col_lds > kLdsLimit)
     return set_error(PS_ERR_UNSUPPORTED, "message needs more LDS than one workgroup has");
   mj.src_kind = kSrcUnary;  // a plain grid, read through the (empty) mask path
-  mj.src = gref(kBufUnary, 0);
+  mj.src = gref(kBufUnary, 0, width);
   mj.src_key = 0;
-  mj.tmp = gref(kBufScratch, 0);
+  mj.tmp = gref(kBufScratch, 0, t_pitch(width));
   mj.dst_kind = kDstUpRoot;
-  mj.dst = gref(kBufMarginal, 0);
+  mj.dst = gref(kBufMarginal, 0, width);
   mj.dst_unary = kNone;
   mj.dst2 = kNone;
+  mj.dst_bel = kNone;
   const long long n64 = ((long long)R * height * t_pitch(width) + 63) & ~63ll;
-  mj.od_buf = mp.od ? gref(kBufScratch, n64) : kNone;
+  mj.od_buf = mp.od ? gref(kBufScratch, n64, 0) : kNone;
   ms.epi = epi_kind(mj);
   MaxJob xj;
   std::memset(&xj, 0, sizeof(xj));
@@ -1538,7 +1586,7 @@ int message_prepare(const ps_model *model, int height, int width, const double o
   xj.W = width;
   xj.src_kind = kSrcUnary;
   xj.key = 0;
-  xj.src = gref(kBufUnary, 0);
+  xj.src = gref(kBufUnary, 0, width);
 
   auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
   ms.o_job = 0;

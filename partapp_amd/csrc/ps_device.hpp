@@ -122,14 +122,19 @@ struct Epi {
   static constexpr bool kNeedU = EPI >= kEpiUpNodeU;  // reads the unary of the part written
   static constexpr bool kNeedB = EPI == kEpiDownMid;  // reads the stored upward belief
   static constexpr bool kDown = EPI >= kEpiDownLeaf;
+  // grids at a second row pitch (byte offset ebs): the unary a chain-node
+  // belief adds (the belief is the plan's grid, the unary the caller's), or
+  // the stored belief and the next source of a five-grid downward epilogue
+  // (the marginal and the unary it adds are at dst's pitch)
+  static constexpr bool kSide = EPI == kEpiUpNodeU || EPI == kEpiDownMid;
   float vmax = -INFINITY;  // max of the written next source / belief
   unsigned best_key = 0u, best_eb = 0u;  // argmax of the marginal (key, byte offset in the slice)
   unsigned second_key = 0u;              // second largest marginal value (key) of the lane
 
-  // one output at byte offset eb of slice r (dst, dst2 point at the slice);
-  // pu / pb are only read when the kind needs them
-  __device__ __forceinline__ void store(float *dst, float *dst2, unsigned eb, float msg, float pu, float pb,
-                                        bool xmask, bool leaf_u) {
+  // one output at byte offset eb of slice r of dst, ebs of dst2 (both point
+  // at the slice); pu / pb are only read when the kind needs them
+  __device__ __forceinline__ void store(float *dst, float *dst2, unsigned eb, unsigned ebs, float msg, float pu,
+                                        float pb, bool xmask, bool leaf_u) {
     auto at = [](float *p, unsigned o) -> float & { return *reinterpret_cast<float *>(reinterpret_cast<char *>(p) + o); };
     if (EPI == kEpiUpRoot || EPI == kEpiUpNode) {
       at(dst, eb) = msg;
@@ -152,10 +157,18 @@ struct Epi {
       }
       if (kNeedB) {
         const float v2 = u + msg;
-        at(dst2, eb) = v2;
+        at(dst2, ebs) = v2;
         vmax = fmaxf(vmax, v2);
       }
     }
+  }
+  // a padding column of the plan's grids (past W, inside the row pitch):
+  // LOG_ZERO, so that every line of such a row is written whole by one wave
+  // (no reduction sees it)
+  __device__ __forceinline__ void pad(float *dst, float *dst2, unsigned eb, unsigned ebs, bool pad_a, bool pad_b) {
+    auto at = [](float *p, unsigned o) -> float & { return *reinterpret_cast<float *>(reinterpret_cast<char *>(p) + o); };
+    if (pad_a) at(dst, eb) = kLogZero;
+    if (kNeedB && pad_b) at(dst2, ebs) = kLogZero;
   }
   // wave reductions and one atomic per wave; every lane of the wave calls it
   __device__ __forceinline__ void flush(const Bases &b, const MsgJob &J, long long slice0, int lane) {
@@ -173,6 +186,14 @@ struct Epi {
   }
 };
 
+// Row pitch of an epilogue's side grids (Epi::kSide): the unary's for a
+// chain-node belief, the next source's (and stored belief's) for the
+// five-grid downward epilogue; dst's otherwise
+template <int EPI>
+__device__ __forceinline__ int epi_pitch_b(const MsgJob &J) {
+  return EPI == kEpiUpNodeU ? J.dst_unary.pitch : (EPI == kEpiDownMid ? J.dst2.pitch : J.dst.pitch);
+}
+
 // Reductions a message's epilogue feeds: the max of the next upward source,
 // the argmax of a marginal and the max of the next downward source.
 struct DstAcc {
@@ -182,16 +203,16 @@ struct DstAcc {
 };
 
 // The caller's accumulation of one message value msg at element
-// flat = r*H*W + y*W + x of slice r, stored at dst[e] / dst2[e] (e = flat, or
-// y*W + x when the pointers already point at slice r).  pu is the unary of
-// the part written (read when need_u), pb the stored upward belief (read when
-// need_b), xmask the part's mask at (r, x).
+// flat = r*H*W + y*W + x of slice r, stored at dst[e] / dst2[e2] (each grid's
+// own element of (r, y, x) at its row pitch).  pu is the unary of the part
+// written (read when need_u), pb the stored upward belief (read when need_b),
+// xmask the part's mask at (r, x).
 //   up, parent = root:  the message slot (objectdetect_findrot.cpp:637-649)
 //   up, chain node:     belief = 0 + message (+ unary) (:637, :652-654)
 //   down:               marginal = belief + message (:201), next downward
 //                       source = unary + message (:220-222)
-__device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *dst2, long long e, unsigned flat,
-                                          bool xmask, float msg, float pu, float pb, DstAcc &a) {
+__device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *dst2, long long e, long long e2,
+                                          unsigned flat, bool xmask, float msg, float pu, float pb, DstAcc &a) {
   if (J.dst_kind == kDstUpRoot) {
     dst[e] = msg;
   } else if (J.dst_kind == kDstUpNode) {
@@ -210,7 +231,7 @@ __device__ __forceinline__ void dst_store(const MsgJob &J, float *dst, float *ds
     a.amax = k > a.amax ? k : a.amax;
     if (dst2) {
       const float v2 = u + msg;
-      dst2[e] = v2;
+      dst2[e2] = v2;
       a.kmax2 = max(a.kmax2, f2key(v2));
     }
   }

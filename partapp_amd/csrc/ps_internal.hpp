@@ -18,9 +18,17 @@ constexpr int kMfmaMaxKK = 8;      // BF16 Toeplitz passes: half widths up to 47
 // Which base pointer an offset is relative to.
 enum BufId : int { kBufUnary = 0, kBufMarginal = 1, kBufScratch = 2, kBufRoot = 3 };
 
+// A grid [R][H][pitch] of floats: the caller's unaries and marginals are
+// dense (pitch W), and so is log_from_root (the root writes it along its
+// pixels); the column passes' own outputs (message slots, upward beliefs,
+// downward sources) start each row on a 128-byte boundary (grid_pitch(W)),
+// and the matrix-core column passes write them over the whole pitch, so that
+// every 128-byte line is written by one wave in one piece: lines written in
+// parts by different waves cost the column passes ~5 % more time than the
+// same bytes in whole lines.
 struct GridRef {
-  int buf;  // BufId, or -1 for "none"
-  int pad_;
+  int buf;    // BufId, or -1 for "none"
+  int pitch;  // row pitch (floats); a slice holds H rows
   long long off;  // element offset into that buffer
 };
 
@@ -80,8 +88,9 @@ struct MsgJob {
   GridRef dst_unary;  // unary of the part written (UpNode: parent, Down: child)
   UnaryMask dst_mask;
   int dst_add_unary;  // UpNode: parent is_detect
-  int dst_base;       // Down: 0 = dst holds the upward belief, 1 = masked unary, 2 = zero
+  int dst_base;       // Down: 0 = the stored upward belief (dst_bel), 1 = masked unary, 2 = zero
   GridRef dst2;       // Down with a grandchild: unary + message (next source), else buf -1
+  GridRef dst_bel;    // Down with a grandchild: the stored upward belief (pitch of dst2), else buf -1
   int dst2_key;
   int map_slot;       // Down: argmax key of the child marginal
   // BF16 Toeplitz variant of the x / y passes: k-steps of 16 and offsets (in
@@ -142,8 +151,10 @@ constexpr int f16_tap_copy_len(int KK) {
 
 // floats of the eigenbasis scratch of one message
 inline long long od_scratch_floats(int R, int th, int tw) { return 3ll * R * th * tw; }
-// row pitch of the row-pass output: rows start on 128-byte boundaries
+// row pitch of the row-pass output and of the plan's own grids: rows start
+// on 128-byte boundaries
 inline int t_pitch(int W) { return (W + 31) & ~31; }
+inline int grid_pitch(int W) { return t_pitch(W); }
 
 // Root of one (instance, scale): sums the upward messages into the root
 // belief, forms the "all but one child" root messages
@@ -154,7 +165,8 @@ struct RootJob {
   int add_unary;
   int inst;
   int pad0_;
-  GridRef msg[kMaxRootChildren];  // upward messages; overwritten with log_from_root
+  GridRef msg[kMaxRootChildren];  // upward messages (the plan's grids, one row pitch for all)
+  GridRef fr[kMaxRootChildren];   // log_from_root of each child (dense grids, as the unary)
   int fr_key[kMaxRootChildren];
   GridRef unary;
   UnaryMask mask;

@@ -301,7 +301,8 @@ __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__r
   const bool zero_src = J.src_kind == kSrcZero;
   const UnaryMask mk = J.src_mask;
   const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
-  int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
+  const int Ps = J.src.pitch;
+  int HW = H * Ps;  // R * H * pitch < 2^31 is checked when the plan is built
   // per OUTPUT slice: the source slice is ro - delta (clamped into [0, R) at
   // so + 3 R), read at (y + oyo, x + oxo) with oxo at so, oyo at so + R;
   // slices without a source carry a row offset that fails the bounds test.
@@ -321,7 +322,7 @@ __device__ __forceinline__ void rowpass_column(const MsgJob &J, const float *__r
 #pragma unroll
   for (int ro = 0; ro < R; ++ro) {
     const int yy = min(max(ys + fwd_oy<R, SPLIT>(shifts, so, ro, ys), 0), H - 1);
-    const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * W) * 4u;
+    const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * Ps) * 4u;
     const int xx4 = min(max(x4 + fwd_ox<R, SPLIT>(shifts, so, ro, x) * 4, 0), xmax4);
     v[ro] = *reinterpret_cast<const float *>(srow + (unsigned)xx4);
   }
@@ -431,7 +432,8 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
   const float negM = -M;
   const float e_fill = expf(kLogZero + negM);
   const float *src = cptr(b, J.src);
-  const long long HW = (long long)H * W;
+  const int Ps = J.src.pitch;
+  const long long HW = (long long)H * Ps;
   const int *fox = shifts + J.shifts;
   const int *foy = fox + R;
   const int n_rot = J.rot_len >> 1;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(512) void k_rowpass_any(Bases b, const MsgJob *__re
           float v;
           if (J.src_kind == kSrcZero) v = 0.0f;
           else if (J.src_kind == kSrcUnary && masked(J.src_mask, rs, xx, W)) v = kLogZero;
-          else v = src[rs * HW + (long long)yy * W + xx];
+          else v = src[rs * HW + (long long)yy * Ps + xx];
           e = __builtin_amdgcn_exp2f((v + negM) * kLog2eF);
         }
       }
@@ -546,7 +548,6 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   const int ybrk = split ? shifts[J.shifts + 14 * R + r] : H;
   const int ystep = split ? shifts[J.shifts + 15 * R + r] : 0;
   const bool col_ok = xin && x + ox >= 0 && x + ox < W;
-  const int HW = H * W;
   int tp = J.tpitch;
   const float *Tr = cptr(b, J.tmp) + (long long)r * H * tp;
   const int ky = J.ky;
@@ -554,9 +555,13 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   const int nrows = col_tile_rows(ky, ROWS);
   const float *fy = taps + J.taps_y;
   const float M = key2f(b.max_keys[J.src_key]);
-  float *dst = mptr(b, J.dst) + r * HW;
-  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
-  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * HW : dst;
+  // row pitches: pa of dst (and of the unary a marginal adds), pb of the
+  // grids at the other pitch (EpiPitch)
+  const int pa = J.dst.pitch, pb = epi_pitch_b<EPI>(J);
+  float *dst = mptr(b, J.dst) + r * H * pa;
+  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * H * J.dst_unary.pitch : dst;
+  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * H * pb : dst;
+  const float *bel = kNeedB ? cptr(b, J.dst_bel) + r * H * pb : dst;
   const bool leaf_u = J.dst_base == 1;  // a leaf's marginal: detected part (else its base is 0)
   const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
   // column loaded by this thread's lane slot (loads outside the grid go to its
@@ -574,7 +579,10 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
   // fetched into registers while tile t is filtered.
   constexpr int kNew = kColTileY / kColWaves;  // new rows per thread
   Epi<EPI> acc;
-  unsigned Wb = (unsigned)W * 4u;
+  // (the padding columns of the plan's grids are left unwritten here: this
+  // pass is bound by its FMAs, and the extra stores cost more than the whole
+  // lines save)
+  unsigned Wb = (unsigned)pa * 4u, Wb2 = (unsigned)pb * 4u;
   for (int seg = 0; seg < (SPLIT ? 2 : 1); ++seg) {
     const int ylo = seg == 0 ? 0 : ybrk, yhi = seg == 0 ? ybrk : H;
     if (ylo >= yhi) continue;
@@ -588,26 +596,29 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
     __syncthreads();
 
     for (int ty0 = ylo; ty0 < yhi; ty0 += kColTileY) {
-      asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
+      asm volatile("" : "+s"(Wb), "+s"(Wb2), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
       const bool more = ty0 + kColTileY < yhi;
       const int y0 = ty0 + wave * kColRows;
-      const unsigned eb0 = (unsigned)(y0 * W + xo) * 4u;
+      const unsigned eb0 = (unsigned)(y0 * pa + xo) * 4u;
+      const unsigned ebs0 = Epi<EPI>::kSide ? (unsigned)(y0 * pb + xo) * 4u : eb0;
       // the unary of the part written, loaded now so that its latency hides
       // behind the filter; the stored belief of the five-grid downward epilogue
       // is loaded after the filter (held through it, its 16 registers make the
       // kernel spill at 128 VGPRs)
-      float pu[kColRows], pb[kColRows];
+      float pu[kColRows], pbel[kColRows];
       if (kNeedU || kNeedB) {
+        // the unary: at the side pitch for a chain-node belief, else at dst's
+        constexpr bool kUs = EPI == kEpiUpNodeU;
         if (ty0 + kColTileY <= H) {
 #pragma unroll
           for (int i = 0; i < kColRows; ++i) {
-            const unsigned eb = eb0 + (unsigned)i * Wb;
+            const unsigned eb = kUs ? ebs0 + (unsigned)i * Wb2 : eb0 + (unsigned)i * Wb;
             if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
           }
         } else {
 #pragma unroll
           for (int i = 0; i < kColRows; ++i) {
-            const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
+            const unsigned eb = (unsigned)(min(y0 + i, H - 1) * (kUs ? pb : pa) + xo) * 4u;
             if (kNeedU) pu[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dun) + eb);
           }
         }
@@ -648,8 +659,8 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
       if (kNeedB) {
 #pragma unroll
         for (int i = 0; i < kColRows; ++i) {
-          const unsigned eb = (unsigned)(min(y0 + i, H - 1) * W + xo) * 4u;
-          pb[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(dst) + eb);
+          const unsigned eb = (unsigned)(min(y0 + i, H - 1) * pb + xo) * 4u;
+          pbel[i] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(bel) + eb);
         }
       }
 #pragma unroll
@@ -658,8 +669,8 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
         // log, +M (:432-435); inverse warp (:437-448): outside -> LOG_ZERO
         const float msg = (col_ok && (unsigned)(y + oy) < (unsigned)H) ? log0(a[i]) + M : kLogZero;
         if (xin && y < yhi)
-          acc.store(dst, dst2, eb0 + (unsigned)i * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask,
-                    leaf_u);
+          acc.store(dst, dst2, eb0 + (unsigned)i * Wb, ebs0 + (unsigned)i * Wb2, msg, kNeedU ? pu[i] : 0.0f,
+                    kNeedB ? pbel[i] : 0.0f, xmask, leaf_u);
       }
       if (more) {
         // move rows [64, nrows) to [0, Lp), then store the prefetched rows
@@ -687,7 +698,7 @@ __device__ __forceinline__ void colpass_strip(const Bases &b, const MsgJob &J, c
       }
     }
   }
-  acc.flush(b, J, (long long)r * HW, lane);
+  acc.flush(b, J, (long long)r * H * pa, lane);
 }
 
 // HI: the highest destination kind among the launch's jobs (the kernel holds
@@ -737,7 +748,14 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
   const int pix = pix0 + threadIdx.x;
   const bool ok = pix < HW;
   const int pc = ok ? pix : HW - 1;  // loads of lanes past the end read the last pixel
-  const int x = pc % W;
+  const int yc = pc / W;
+  const int x = pc - yc * W;
+  // the lanes walk the dense grids (unary, marginal, log_from_root, root
+  // posterior: index r HW + pix), so each wave writes its rows' lines whole;
+  // the upward messages are read from the plan's slots (row pitch pm)
+  const int pm = J.msg[0].pitch;
+  const int HPm = H * pm;
+  const int pcm = yc * pm + x;
   const int K = KC <= 8 ? KC : J.K;
   const float *un = cptr(b, J.unary);
   float *marg = mptr(b, J.marginal);
@@ -756,9 +774,9 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
   // stores could alias them as far as the compiler knows)
   float m[KC], u;
   auto load = [&](int r, float (&mm)[KC], float &uu) {
-    const int e = r * HW + pc;
+    const int e = r * HW + pc, em = r * HPm + pcm;
 #pragma unroll
-    for (int k = 0; k < KC; ++k) mm[k] = k < K ? cptr(b, J.msg[k])[e] : 0.0f;
+    for (int k = 0; k < KC; ++k) mm[k] = k < K ? cptr(b, J.msg[k])[em] : 0.0f;
     uu = un[e];
   };
   load(0, m, u);
@@ -793,7 +811,7 @@ __device__ __forceinline__ void root_pixels(const Bases &b, const RootJob &J, in
           if (k < K && k != i) fr = fr + m[k];
         fr = fr + uu;
         if (ok) {
-          __builtin_nontemporal_store(fr, mptr(b, J.msg[i]) + e);
+          __builtin_nontemporal_store(fr, mptr(b, J.fr[i]) + e);
           kmax[i] = max(kmax[i], f2key(fr));
         }
       }

@@ -193,7 +193,8 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
   const bool zero_src = J.src_kind == kSrcZero;
   const UnaryMask mk = J.src_mask;
   const bool use_mask = J.src_kind == kSrcUnary && (mk.rot_bits != 0ull || mk.strip != 0);
-  int HW = H * W;  // R * H * W < 2^31 is checked when the plan is built
+  int Ps = J.src.pitch;  // the source's row pitch (the caller's unaries: W; the plan's grids: padded)
+  int HW = H * Ps;       // R * H * pitch < 2^31 is checked when the plan is built
   int delta = J.delta;
   // forward offsets by output slice (the source slice is ro - delta): x at
   // so, y at so + R, the source slice clamped into [0, R) at so + 3 R; read
@@ -222,12 +223,12 @@ __device__ __forceinline__ void rowpass_mfma_body(Bases b, const MsgJob *__restr
     }
   };
   auto gather = [&](int y, float (&v)[R]) {
-    asm volatile("" : "+s"(so), "+s"(HW));
+    asm volatile("" : "+s"(so), "+s"(HW), "+s"(Ps));
     const int x4 = lane_column() * 4, xmax4 = (W - 1) * 4;
 #pragma unroll
     for (int ro = 0; ro < R; ++ro) {
       const int yy = min(max(y + shifts[so + R + ro], 0), H - 1);
-      const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * W) * 4u;
+      const char *srow = reinterpret_cast<const char *>(src) + (unsigned)(shifts[so + 3 * R + ro] * HW + yy * Ps) * 4u;
       const int xx4 = min(max(x4 + shifts[so + ro] * 4, 0), xmax4);
       v[ro] = *reinterpret_cast<const float *>(srow + (unsigned)xx4);
     }
@@ -541,10 +542,13 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   const int ox = shifts[J.shifts + 2 * R + r];
   const int oy = shifts[J.shifts + 3 * R + r];
   const float M = key2f(b.max_keys[J.src_key]);
-  const int HW = H * W;
-  float *dst = mptr(b, J.dst) + r * HW;
-  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
-  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * HW : dst;
+  // row pitches: pa of dst (and of the unary a marginal adds), pitch_b of
+  // the grids at the other pitch (Epi::kSide)
+  const int pa = J.dst.pitch, pitch_b = epi_pitch_b<EPI>(J);
+  float *dst = mptr(b, J.dst) + r * H * pa;
+  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * H * J.dst_unary.pitch : dst;
+  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * H * pitch_b : dst;
+  const float *bel = kNeedB ? cptr(b, J.dst_bel) + r * H * pitch_b : dst;
   const bool leaf_u = J.dst_base == 1;  // kEpiDownLeaf: a detected part (else its base is 0)
   const UnaryMask dm = J.dst_mask;
 
@@ -642,34 +646,44 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
   constexpr float kLn2 = 0.693147180559945309f;
   const float cM = fmaf(cexp[cx + n], kLn2, M);
   const int xo = xin ? x : W - 1;
-  unsigned Wb = (unsigned)W * 4u;
+  // the plan's own grids are written over their whole row pitch (LOG_ZERO in
+  // the padding columns [W, pitch)): a 128-byte line written in parts by
+  // different waves costs more than its bytes.  Loads keep the clamped column.
+  const bool pad_a = !xin && x < pa;                           // a padding column of dst
+  const bool pad_b = Epi<EPI>::kNeedB && !xin && x < pitch_b;  // of the next source
+  const int xa = pad_a ? x : xo, xb = pad_b ? x : xo;
+  unsigned Wb = (unsigned)pa * 4u, Wb2 = (unsigned)pitch_b * 4u;
+  constexpr bool kUs = EPI == kEpiUpNodeU;  // the unary at the side pitch (a chain-node belief)
 
   Epi<EPI> acc;
   int base = 0;
   for (int ty0 = 0; ty0 < H; ty0 += 64) {
     // the per-row offsets are recomputed every tile rather than hoisted into
     // registers for the whole loop
-    asm volatile("" : "+s"(Wb), "+s"(tp));
+    asm volatile("" : "+s"(Wb), "+s"(Wb2), "+s"(tp));
     // epilogue operands of this lane's 16 outputs, rows yl + 8 (i / 4) + i % 4,
     // at 32-bit byte offsets from the slice (rows past the grid's end, in the
     // last tile, read its last row)
     const int yl = ty0 + ry + 4 * h;
-    const unsigned eb0 = (unsigned)(yl * W + xo) * 4u;
+    const unsigned eb0 = (unsigned)(yl * pa + xa) * 4u;
+    const unsigned ebs0 = Epi<EPI>::kSide ? (unsigned)(yl * pitch_b + xb) * 4u : eb0;
     float pu[16], pb[16];
     if (kNeedU || kNeedB) {
       if (ty0 + 64 <= H) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const unsigned eb = eb0 + (unsigned)(8 * (i >> 2) + (i & 3)) * Wb;
-          if (kNeedU) pu[i] = at(dun, eb);
-          if (kNeedB) pb[i] = at(dst, eb);
+          const unsigned dy = (unsigned)(8 * (i >> 2) + (i & 3));
+          const unsigned eb = eb0 + dy * Wb, ebs = ebs0 + dy * Wb2;
+          if (kNeedU) pu[i] = at(dun, kUs ? ebs : eb);
+          if (kNeedB) pb[i] = at(bel, ebs);
         }
       } else {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const unsigned eb = (unsigned)(min(yl + 8 * (i >> 2) + (i & 3), H - 1) * W + xo) * 4u;
-          if (kNeedU) pu[i] = at(dun, eb);
-          if (kNeedB) pb[i] = at(dst, eb);
+          const int yc = min(yl + 8 * (i >> 2) + (i & 3), H - 1);
+          const unsigned eb = (unsigned)(yc * pa + xa) * 4u, ebs = (unsigned)(yc * pitch_b + xb) * 4u;
+          if (kNeedU) pu[i] = at(dun, kUs ? ebs : eb);
+          if (kNeedB) pb[i] = at(bel, ebs);
         }
       }
     }
@@ -704,7 +718,10 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
       const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
       const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
       if (xin && y < H)
-        acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask, leaf_u);
+        acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, ebs0 + (unsigned)dy * Wb2, msg, kNeedU ? pu[i] : 0.0f,
+                  kNeedB ? pb[i] : 0.0f, xmask, leaf_u);
+      else if ((pad_a || pad_b) && y < H)
+        acc.pad(dst, dst2, eb0 + (unsigned)dy * Wb, ebs0 + (unsigned)dy * Wb2, pad_a, pad_b);
     }
     if (more) {
       __syncthreads();  // every wave is done with this tile's ring rows
@@ -716,7 +733,7 @@ __device__ __forceinline__ void colpass_mfma_strip(const Bases &b, const MsgJob 
       __syncthreads();
     }
   }
-  acc.flush(b, J, (long long)r * HW, lane);
+  acc.flush(b, J, (long long)r * H * pa, lane);
 }
 
 // ---------------------------------------- column pass on an fp16 row-pass output
@@ -783,10 +800,13 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   } else {
     emax = rexp[0];
   }
-  const int HW = H * W;
-  float *dst = mptr(b, J.dst) + r * HW;
-  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * HW : dst;
-  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * HW : dst;
+  // row pitches: pa of dst (and of the unary a marginal adds), pitch_b of
+  // the grids at the other pitch (Epi::kSide)
+  const int pa = J.dst.pitch, pitch_b = epi_pitch_b<EPI>(J);
+  float *dst = mptr(b, J.dst) + r * H * pa;
+  const float *dun = kNeedU ? cptr(b, J.dst_unary) + r * H * J.dst_unary.pitch : dst;
+  float *dst2 = kNeedB ? mptr(b, J.dst2) + r * H * pitch_b : dst;
+  const float *bel = kNeedB ? cptr(b, J.dst_bel) + r * H * pitch_b : dst;
   const bool leaf_u = J.dst_base == 1;
   const UnaryMask dm = J.dst_mask;
 
@@ -859,31 +879,41 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
   constexpr float kLn2 = 0.693147180559945309f;
   const float cM = (float)(emax - J.t16_shift) * kLn2;
   const int xo = xin ? x : W - 1;
-  unsigned Wb = (unsigned)W * 4u;
+  // the plan's own grids are written over their whole row pitch (LOG_ZERO in
+  // the padding columns [W, pitch)): a 128-byte line written in parts by
+  // different waves costs more than its bytes.  Loads keep the clamped column.
+  const bool pad_a = !xin && x < pa;                           // a padding column of dst
+  const bool pad_b = Epi<EPI>::kNeedB && !xin && x < pitch_b;  // of the next source
+  const int xa = pad_a ? x : xo, xb = pad_b ? x : xo;
+  unsigned Wb = (unsigned)pa * 4u, Wb2 = (unsigned)pitch_b * 4u;
+  constexpr bool kUs = EPI == kEpiUpNodeU;  // the unary at the side pitch (a chain-node belief)
 
   Epi<EPI> acc;
   int base = 0;
   const unsigned char *kp = PRUNE ? b.keep + J.keep_off + (long long)r * J.keep_ty * J.keep_tx + xt : nullptr;
   for (int ty0 = 0; ty0 < H; ty0 += 64) {
-    asm volatile("" : "+s"(Wb), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
+    asm volatile("" : "+s"(Wb), "+s"(Wb2), "+s"(tp));  // per-row offsets recomputed per tile, not hoisted
     const int yl = ty0 + ry + 4 * h;
-    const unsigned eb0 = (unsigned)(yl * W + xo) * 4u;
+    const unsigned eb0 = (unsigned)(yl * pa + xa) * 4u;
+    const unsigned ebs0 = Epi<EPI>::kSide ? (unsigned)(yl * pitch_b + xb) * 4u : eb0;
     const bool kept = !PRUNE || kp[(ty0 >> 6) * J.keep_tx] != 0;
     float pu[16], pb[16];
     if ((kNeedU || kNeedB) && kept) {
       if (ty0 + 64 <= H) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const unsigned eb = eb0 + (unsigned)(8 * (i >> 2) + (i & 3)) * Wb;
-          if (kNeedU) pu[i] = at(dun, eb);
-          if (kNeedB) pb[i] = at(dst, eb);
+          const unsigned dy = (unsigned)(8 * (i >> 2) + (i & 3));
+          const unsigned eb = eb0 + dy * Wb, ebs = ebs0 + dy * Wb2;
+          if (kNeedU) pu[i] = at(dun, kUs ? ebs : eb);
+          if (kNeedB) pb[i] = at(bel, ebs);
         }
       } else {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-          const unsigned eb = (unsigned)(min(yl + 8 * (i >> 2) + (i & 3), H - 1) * W + xo) * 4u;
-          if (kNeedU) pu[i] = at(dun, eb);
-          if (kNeedB) pb[i] = at(dst, eb);
+          const int yc = min(yl + 8 * (i >> 2) + (i & 3), H - 1);
+          const unsigned eb = (unsigned)(yc * pa + xa) * 4u, ebs = (unsigned)(yc * pitch_b + xb) * 4u;
+          if (kNeedU) pu[i] = at(dun, kUs ? ebs : eb);
+          if (kNeedB) pb[i] = at(bel, ebs);
         }
       }
     }
@@ -911,16 +941,18 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
         const bool ok = col_ok && (unsigned)(y + oy) < (unsigned)H && lg > -INFINITY;
         const float msg = ok ? fmaf(lg, kLn2, cM) : kLogZero;
         if (xin && y < H)
-          acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, msg, kNeedU ? pu[i] : 0.0f, kNeedB ? pb[i] : 0.0f, xmask,
-                    leaf_u);
+          acc.store(dst, dst2, eb0 + (unsigned)dy * Wb, ebs0 + (unsigned)dy * Wb2, msg, kNeedU ? pu[i] : 0.0f,
+                    kNeedB ? pb[i] : 0.0f, xmask, leaf_u);
+        else if ((pad_a || pad_b) && y < H)
+          acc.pad(dst, dst2, eb0 + (unsigned)dy * Wb, ebs0 + (unsigned)dy * Wb2, pad_a, pad_b);
       }
     } else {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int dy = 8 * (i >> 2) + (i & 3);
-        if (xin && yl + dy < H) {
-          at(dst, eb0 + (unsigned)dy * Wb) = kLogZero;
-          if (kNeedB) at(dst2, eb0 + (unsigned)dy * Wb) = kLogZero;
+        if (yl + dy < H) {
+          if (xin || pad_a) at(dst, eb0 + (unsigned)dy * Wb) = kLogZero;
+          if (kNeedB && (xin || pad_b)) at(dst2, ebs0 + (unsigned)dy * Wb2) = kLogZero;
         }
       }
     }
@@ -934,7 +966,7 @@ __device__ __forceinline__ void colpass_f16_strip(const Bases &b, const MsgJob &
       __syncthreads();
     }
   }
-  acc.flush(b, J, (long long)r * HW, lane);
+  acc.flush(b, J, (long long)r * H * pa, lane);
 }
 
 // HI: the highest epilogue kind among the launch's jobs.  The kernel holds

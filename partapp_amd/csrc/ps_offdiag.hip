@@ -227,6 +227,10 @@ __global__ __launch_bounds__(kOdThreads) void k_od_back(Bases b, const MsgJob *_
   float *dst = mptr(b, J.dst);
   const float *dun = J.dst_unary.buf >= 0 ? cptr(b, J.dst_unary) : nullptr;
   float *dst2 = J.dst2.buf >= 0 ? mptr(b, J.dst2) : nullptr;
+  const float *bel = J.dst_bel.buf >= 0 ? cptr(b, J.dst_bel) : nullptr;
+  // element (r, y, x) of a grid at its row pitch (i is the dense one)
+  const long long ry = (long long)r * H + y;
+  auto at = [&](const GridRef &g) { return ry * g.pitch + x; };
   DstAcc da;
   if (ok) {
     float msg = kLogZero;
@@ -238,10 +242,10 @@ __global__ __launch_bounds__(kOdThreads) void k_od_back(Bases b, const MsgJob *_
     }
     const bool need_u = (J.dst_kind == kDstUpNode && J.dst_add_unary) || J.dst_kind == kDstDown;
     const bool need_b = J.dst_kind == kDstDown && J.dst_base == 0;
-    const float pu = need_u ? dun[i] : 0.0f;
-    const float pb = need_b ? dst[i] : 0.0f;
+    const float pu = need_u ? dun[at(J.dst_unary)] : 0.0f;
+    const float pb = need_b ? bel[at(J.dst_bel)] : 0.0f;
     const bool xmask = ((J.dst_mask.rot_bits >> r) & 1ull) || x < J.dst_mask.strip || x >= W - J.dst_mask.strip;
-    dst_store(J, dst, dst2, i, (unsigned)i, xmask, msg, pu, pb, da);
+    dst_store(J, dst, dst2, at(J.dst), dst2 ? at(J.dst2) : 0, (unsigned)i, xmask, msg, pu, pb, da);
   }
   dst_flush(b, J, dst2 != nullptr, da, threadIdx.x & 63);
 }

@@ -559,13 +559,12 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       auto bel = [&](int p) {
         return gref(kBufScratch, (long long)scratch_off + K * nt64 + (K + inner_idx[p]) * np64, gp);
       };
-      // log_from_root of chain k (written by the root combine, read by the
-      // first downward message of the chain): dense like the caller's grids,
-      // so the root writes all its grids along its pixels
-      const long long n64 = (n + 63) & ~63ll;
-      auto lfr = [&](int k) {
-        return gref(kBufScratch, (long long)scratch_off + K * nt64 + (K + n_inner) * np64 + (long long)k * n64, W);
-      };
+      // log_from_root of chain k: written by the root combine into the
+      // marginal grid of the chain's first part (dense, so the root writes all
+      // its grids along its pixels), read by the row pass of the chain's first
+      // downward message, whose column pass then writes that part's marginal
+      // over it
+      auto lfr = [&](int k) { return marg(chains[k][0]); };
       auto up_key = [&](int p) { return it * 2 * P + p; };
       auto dn_key = [&](int p) { return it * 2 * P + P + p; };
       auto slot = [&](int p) { return it * P + p; };
@@ -616,7 +615,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
       std::vector<long long> od_off(K, 0);
       long long od_total = 0;
       for (int k = 0; k < K; ++k) {
-        od_off[k] = (long long)scratch_off + K * nt64 + (K + n_inner) * np64 + K * n64 + od_total;
+        od_off[k] = (long long)scratch_off + K * nt64 + (K + n_inner) * np64 + od_total;
         od_total += od_floats[k];
       }
       auto odbuf = [&](int k) { return gref(kBufScratch, od_off[k], 0); };
@@ -744,7 +743,7 @@ extern "C" int ps_plan_create(const ps_model *model, const ps_instance *instance
         mj.out = it * P + p;
         B.maps.push_back(mj);
       }
-      scratch_off += (size_t)(K * nt64 + (K + n_inner) * np64 + K * n64 + od_total);
+      scratch_off += (size_t)(K * nt64 + (K + n_inner) * np64 + od_total);
     }
   }
 

@@ -19,13 +19,13 @@ constexpr int kMfmaMaxKK = 8;      // BF16 Toeplitz passes: half widths up to 47
 enum BufId : int { kBufUnary = 0, kBufMarginal = 1, kBufScratch = 2, kBufRoot = 3 };
 
 // A grid [R][H][pitch] of floats: the caller's unaries and marginals are
-// dense (pitch W), and so is log_from_root (the root writes it along its
-// pixels); the column passes' own outputs (message slots, upward beliefs,
-// downward sources) start each row on a 128-byte boundary (grid_pitch(W)),
-// and the matrix-core column passes write them over the whole pitch, so that
-// every 128-byte line is written by one wave in one piece: lines written in
-// parts by different waves cost the column passes ~5 % more time than the
-// same bytes in whole lines.
+// dense (pitch W), and so is log_from_root (held in a marginal grid until the
+// downward pass writes the marginal); the column passes' own outputs
+// (message slots, upward beliefs, downward sources) start each row on a
+// 128-byte boundary (grid_pitch(W)), and the matrix-core column passes write
+// them over the whole pitch, so that every 128-byte line is written by one
+// wave in one piece: lines written in parts by different waves cost the
+// column passes ~5 % more time than the same bytes in whole lines.
 struct GridRef {
   int buf;    // BufId, or -1 for "none"
   int pitch;  // row pitch (floats); a slice holds H rows
@@ -166,7 +166,7 @@ struct RootJob {
   int inst;
   int pad0_;
   GridRef msg[kMaxRootChildren];  // upward messages (the plan's grids, one row pitch for all)
-  GridRef fr[kMaxRootChildren];   // log_from_root of each child (dense grids, as the unary)
+  GridRef fr[kMaxRootChildren];   // log_from_root of each child (dense: the child's marginal grid)
   int fr_key[kMaxRootChildren];
   GridRef unary;
   UnaryMask mask;

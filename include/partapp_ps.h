@@ -117,8 +117,12 @@ size_t ps_plan_scratch_bytes(const ps_plan *plan);
 /* Launch the whole sweep on `stream` (a hipStream_t, NULL = default stream).
  * log_unary is read-only: the upright / border masks of
  * objectdetect_findrot.cpp:509-551 are applied on the fly instead of being
- * written into the caller's grids.  map_dev: [n_instances][S][P] ps_map in
- * device memory.  Asynchronous.  root_posterior may be NULL. */
+ * written into the caller's grids.  log_marginal also holds intermediate
+ * grids while the sweep runs (a part's marginal grid carries the root's
+ * message to it until that part's marginal is written); it holds the
+ * marginals once the sweep's work on the stream is done.  map_dev:
+ * [n_instances][S][P] ps_map in device memory.  Asynchronous.
+ * root_posterior may be NULL. */
 int ps_plan_execute(ps_plan *plan, const float *log_unary, float *log_marginal, float *root_posterior,
                     ps_map *map_dev, void *stream);
 /* Same, through a HIP graph captured on first use for this pointer set

@@ -9,6 +9,10 @@
 * The marginal grids carry intermediate grids during a sweep: a second
   execution on the same buffers (holding the first one's results) gives the
   same outputs bit for bit, as does one on buffers filled with other values.
+* Narrow widths, where a column strip is mostly the padding columns of the
+  plan's grids (written as LOG_ZERO, never read as data): FP32 against the
+  float32 oracle at its bar, bf16mix against FP64 at the method's bar (1e-3)
+  with the MAP guard on (every MAP equal).
 """
 import sys
 
@@ -71,3 +75,38 @@ def test_marginal_buffers_start_state_does_not_matter(gpu, method):
     for a, b in zip((marg, root, maps), ref):
         assert torch.equal(a, b)
     assert torch.isfinite(marg).all()
+
+
+# the last column strip holds 1, 33 and 1 columns of the grid and 31 padding
+# columns (grids narrower than the joints' offsets of up to 40 px would move
+# whole messages out of the grid)
+NARROW = [(96, 65), (80, 97), (88, 129)]
+
+
+def test_narrow_widths_fp32(gpu):
+    from test_gpu_parity import check_parity, run_plan
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = [synth_joints(430 + i) for i in range(len(NARROW))]
+    model = person_model(ep, joints[0])
+    us = [synth_unary(440 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(NARROW)]
+    say("FP32 on widths", [w for _, w in NARROW])
+    outs = run_plan(model, us, joints)
+    for i, u in enumerate(us):
+        check_parity("%d x %d" % NARROW[i], model, u, joints[i], outs[i], check_f64=False)
+
+
+def test_narrow_widths_bf16mix(gpu):
+    from test_gpu_bf16 import BUDGET_L1, errors_vs_f64, run_method
+    ep = ExpParam(-180.0, 180.0, 24, 1.0, 1.0, 1, 0.0)
+    joints = [synth_joints(450 + i) for i in range(len(NARROW))]
+    model = person_model(ep, joints[0])
+    us = [synth_unary(460 + i, 10, 1, 24, h, w) for i, (h, w) in enumerate(NARROW)]
+    say("bf16mix on widths", [w for _, w in NARROW])
+    outs, _ = run_method(model, us, joints, "bf16mix")
+    for i, u in enumerate(us):
+        worst, map_bad = errors_vs_f64(model, u, joints[i], outs[i])
+        say("  %d x %d: worst rel L1 vs f64 %.3g, MAP mismatches %s" % (NARROW[i] + (worst, map_bad)))
+        # the method's bar (these small grids reach 4.8e-4 / 8.4e-4 / 3.3e-4,
+        # the same as with dense rows): padding read as data would be far off
+        assert worst <= BUDGET_L1
+        assert not map_bad
